@@ -1,0 +1,217 @@
+"""Throughput of the gfx950 per-timestep path: MLUPS and % of the HBM roofline.
+
+Default workload: D3Q27 + D3Q7 two-phase jet, FP32, 128x128x512 (inlet
+diameter 16, u_z = 0.05, sigma = 0.05), W warm-up steps then K timed
+steps replayed from HIP graphs (defaults 5000 + 5000: throughput over the
+last 5000 of 10000 steps).  MLUPS counts every cell, ghosts included, as
+the reference does (main.cu:199-203).
+
+Also measured in the same run and reported in the same JSON line:
+  * the fused lattice kernel's average duration from HIP events on the
+    stream it runs on -> roofline.achieved (algorithmic bytes / time);
+  * the 16-bit storage form against its FP32 twin (D3Q19 + D3Q7,
+    256x256x1024), as the metric asks for FP16 storage vs FP32;
+  * the CPU baseline: the oracle's restatement of the reference step in
+    FP32 with OpenMP on a bounded sample of the same workload.
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS + % HBM roofline, D3Q19/27+D3Q7 jet; FP16-storage vs FP32"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measure(case, steps: int, warmup: int, kernel_steps: int):
+    import torch
+
+    from ssmdlbm_amd import cases
+    from ssmdlbm_amd.solver import Simulation
+
+    p = case.params()
+    sim = Simulation(p, "cuda:0")
+    sim.init()
+    interior = (p.nx - 2) * (p.ny - 2) * (p.nz - 2)
+    if warmup:
+        sim.graph_run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.graph_run(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    mlups = p.cells * steps / wall / 1e6
+
+    # fused lattice kernel alone, bracketed by events on its own stream
+    stream = sim.stream
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(kernel_steps)]
+    for e0, e1 in ev:
+        sim.launch_phase()
+        e0.record(stream)
+        sim.launch_lattice()
+        e1.record(stream)
+        sim.launch_boundaries()
+        sim.swap()
+    torch.cuda.synchronize()
+    times = sorted(e0.elapsed_time(e1) for e0, e1 in ev)
+    k_ms = sum(times) / len(times)
+    kb = cases.lattice_kernel_bytes_per_cell(case.q, case.pop_format)
+    achieved = interior * kb / (k_ms * 1e-3) / 1e9
+    step_bytes = cases.algorithmic_bytes_per_cell(case.q, case.pop_format)
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    return {
+        "mlups": mlups,
+        "ms_per_step": wall / steps * 1e3,
+        "kernel_ms": k_ms,
+        "kernel_ms_median": times[len(times) // 2],
+        "kernel_gbs": achieved,
+        "kernel_bytes_per_cell": kb,
+        "interior_cells": interior,
+        "step_bytes_per_cell": step_bytes,
+        "step_frac": mlups * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9),
+    }
+
+
+def cpu_baseline(case, budget_s: float = 15.0):
+    from oracle.oracle import Oracle
+
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(16, threads))
+    o = Oracle(case.params(), fp64=False, threads=threads)
+    o.init()
+    t0 = time.perf_counter()
+    o.step(2)
+    per = (time.perf_counter() - t0) / 2
+    n = int(max(3, min(200, budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(n)
+    wall = time.perf_counter() - t0
+    return {
+        "value": round(case.cells * n / wall / 1e6, 3),
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{case.name}: {n} timed steps after 2 warm-up steps of the oracle's FP32 restatement of the "
+                  f"reference schedule (K1-K4 + boundaries), OpenMP over z, {threads} threads",
+    }
+
+
+def load_traffic(workload: str):
+    path = ROOT / "profiles" / f"k_lattice_{workload}_traffic.json"
+    if path.exists():
+        try:
+            return json.loads(path.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=5000)
+    ap.add_argument("--workload", default="jet_d3q27_128x128x512")
+    ap.add_argument("--kernel-steps", type=int, default=200)
+    ap.add_argument("--form-steps", type=int, default=300)
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP16-vs-FP32 storage measurement")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the multi-GPU z-slab decomposition is a design, not code)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from ssmdlbm_amd import abi, cases
+
+    if not torch.cuda.is_available():
+        print("bench.py needs the GPU", file=sys.stderr)
+        return 1
+    abi.load()
+    case = cases.WORKLOADS[args.workload]
+    log(f"workload {case.name}: warmup {args.warmup}, timed {args.steps}")
+    main_m = measure(case, args.steps, args.warmup, args.kernel_steps)
+    log(f"main: {main_m}")
+
+    forms = None
+    if not args.no_forms:
+        forms = {}
+        for c in (cases.JET_D3Q19_FP32_LARGE, cases.JET_D3Q19_FP16):
+            m = measure(c, args.form_steps, 50, min(args.kernel_steps, 100))
+            forms[c.name] = {k: round(v, 4) if isinstance(v, float) else v for k, v in m.items()}
+            log(f"form {c.name}: {m}")
+        a = forms[cases.JET_D3Q19_FP16.name]["mlups"]
+        b = forms[cases.JET_D3Q19_FP32_LARGE.name]["mlups"]
+        forms["fp16_over_fp32_mlups"] = round(a / b, 4)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(case)
+        log(f"cpu: {cpu}")
+
+    kb = main_m["kernel_bytes_per_cell"]
+    out = {
+        "metric": METRIC,
+        "value": round(main_m["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_m["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if case.pop_format == abi.POP_F32 else ("f16-storage/f32" if case.pop_format == abi.POP_F16
+                                                                else "bf16-storage/f32"),
+        "data": "synthetic",
+        "config": {
+            "workload": case.name,
+            "lattice": f"D3Q{case.q}+D3Q7",
+            "box": [case.nx, case.ny, case.nz],
+            "tau": case.tau, "tau_g": case.tau_g, "sigma": case.sigma, "inlet_diameter": case.diameter,
+            "u_inlet": case.u, "perturb": case.perturb,
+            "timed_region": "HIP graph replay, one graph per step (phase, fused lattice, boundary kernels)",
+            "mlups_cells": "all cells incl. ghost layer (reference convention)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(main_m["kernel_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(main_m["kernel_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(case.name),
+            "kernel": "k_lattice (fused computeForces+computeMoments+streamCollide)",
+            "kernel_ms": round(main_m["kernel_ms"], 5),
+            "algorithmic_bytes_per_launch": main_m["interior_cells"] * kb,
+            "step_frac_at_B_alg": round(main_m["step_frac"], 4),
+        },
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,190 @@
+/*
+ * ssmd.h -- C ABI of the MI355X (gfx950) per-timestep hot path of the
+ * ssmdLBM two-phase jet solver: regularized D3Q19 / D3Q27 stream+collide,
+ * the D3Q7 conservative Allen-Cahn phase field (phi, normals, curvature,
+ * surface-tension force) and the jet inlet / outflow / periodic boundaries.
+ *
+ * Reference interface this header replaces (file:line in brenogemelgo/ssmdLBM):
+ *   - struct LBMFields               src/structs/LBMFields.cuh:26-85
+ *   - host::FieldAllocate            src/fieldAllocate/FieldAllocate.cuh:42-162
+ *   - kernel set k(LBMFields)        src/LBMIncludes.cuh:29-47, src/phaseField.cuh:31,54
+ *   - graph::captureGraph            src/cuda/CUDAGraph.cuh:34-57
+ *   - compile-time constants         src/constants.cuh:70-101, src/functions/globalFunctions.cuh:55-104,232-283
+ *
+ * Conventions
+ *   - Every function returns 0 on success, a negative SSMD_E* code for an
+ *     argument error, or a positive hipError_t value from the HIP runtime.
+ *     Nothing aborts inside the library (the reference aborts in its
+ *     checkCudaErrors macros, src/cuda/utils.cuh:101-161; a host driver that
+ *     wants that behaviour aborts on a nonzero return).
+ *   - All device work is asynchronous on the caller's stream (`stream` is a
+ *     hipStream_t passed as void*; NULL selects the null stream).  One field
+ *     set per stream; calls on one field set must not overlap.
+ *   - Scalar fields are float32, x fastest: idx = x + nx*(y + ny*z), ghost
+ *     layer = the outermost plane on every face (reference deviceFunctions.cuh:33-59).
+ *   - Populations are Q-major planes, f[i*N + idx], stored as deviations
+ *     fbar_i = f_i - w_i (reference D3Q19.cuh:139).  Unlike the reference's
+ *     arrival-indexed push layout, the buffers here are SOURCE-indexed
+ *     ("pull"): slot f[i][s] holds the post-collision population that leaves
+ *     cell s along c_i, so the population arriving at cell a along c_i is
+ *     f[i][a - c_i].  Ghost-cell slots hold what the boundaries emit.
+ *     Two buffer pairs ping-pong; ssmd_step swaps f<->f_next, g<->g_next.
+ */
+#ifndef SSMD_H
+#define SSMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSMD_ABI_VERSION 1
+
+#define SSMD_OK 0
+#define SSMD_EINVAL (-1)       /* bad argument or inconsistent parameters */
+#define SSMD_EUNSUPPORTED (-2) /* parameter combination not built */
+#define SSMD_ENOTREADY (-3)    /* fields not allocated / graph not created */
+
+/* Population storage formats.  F32 is the reference's default scalar_t
+ * storage; F16 is its ENABLE_FP16 pop_t (cuda/utils.cuh:67-99, IEEE binary16
+ * with round-to-nearest-even); BF16 is new. Arithmetic is always float32. */
+enum { SSMD_POP_F32 = 0, SSMD_POP_F16 = 1, SSMD_POP_BF16 = 2 };
+
+/* Periodic handling of an axis.
+ *   NONE      : no wrap; populations leaving from ghost cells stay frozen.
+ *   REFERENCE : the reference periodicX copy (BoundaryConditions.cuh:295-349):
+ *               arrival planes 1 <-> n-2 exchange, phi/u ghost wrap.  For y
+ *               it is the mirror image of that copy.
+ *   TRUE      : populations re-enter with period n-2 (mass conserving);
+ *               phi/u ghost wrap as REFERENCE. */
+enum { SSMD_WRAP_NONE = 0, SSMD_WRAP_REFERENCE = 1, SSMD_WRAP_TRUE = 2 };
+
+/* ssmd_step flags */
+#define SSMD_STEP_MACROS 1u /* write rho, u, Pi and F at every interior cell */
+
+/* A circular velocity inlet on a ghost face.
+ * water: face y = 0, disc coordinates (a, b) = (x, z)   (applyWaterInflow, BoundaryConditions.cuh:36-99)
+ * oil:   face z = 0, disc coordinates (a, b) = (x, y)   (applyOilInflow,   BoundaryConditions.cuh:101-168) */
+typedef struct ssmd_jet {
+    int32_t enabled;
+    int32_t pad0;
+    double center_a; /* geometry::center_x() = (nx-1)/2 in the reference */
+    double center_b; /* geometry::z_pos() / y_pos() in the reference */
+    double r2;       /* geometry::R2_*(): (D/2)^2 with integer D/2 */
+    double u;        /* inflow speed along the face normal (physics::u_*) */
+    double phi;      /* phase carried in: 0 water, 1 oil */
+} ssmd_jet;
+
+typedef struct ssmd_params {
+    uint32_t nx, ny, nz; /* mesh::nx, ny, nz (ghost layer included) */
+    int32_t q;           /* 19 or 27: hydrodynamic lattice */
+    int32_t pop_format;  /* SSMD_POP_* */
+    int32_t outflow_y;   /* applyOutflowY on y = ny-1 */
+    int32_t outflow_z;   /* applyOutflowZ on z = nz-1 */
+    int32_t wrap_x, wrap_y, wrap_z; /* SSMD_WRAP_* */
+    double visc_water;   /* relaxation::visc_water() */
+    double visc_oil;     /* relaxation::visc_oil() */
+    double sigma;        /* physics::sigma */
+    double gamma;        /* physics::gamma (D3Q7 sharpening) */
+    double perturb;      /* inlet velocity noise: each step, cell and component
+                            adds perturb * u * U[-1,1) (0 in the reference) */
+    uint64_t seed;       /* seed of that noise */
+    ssmd_jet water;
+    ssmd_jet oil;
+} ssmd_params;
+
+/* Device buffers.  Member order follows struct LBMFields (LBMFields.cuh:26-85);
+ * normx/normy/normz/ind are never allocated, as in the reference (main.cu:55-70).
+ * rho, u and Pi are refreshed every step on the six planes next to the faces
+ * (x,y,z in {1, n-2}: the boundary kernels read them there); everywhere else
+ * only on steps run with SSMD_STEP_MACROS, which also writes fsx/fsy/fsz. */
+typedef struct ssmd_fields {
+    float *rho, *ux, *uy, *uz;
+    float *pxx, *pyy, *pzz, *pxy, *pxz, *pyz;
+    float *phi;
+    float *normx, *normy, *normz, *ind;
+    float *fsx, *fsy, *fsz;
+    void *f;          /* populations the next step reads (pop_format elements) */
+    float *g;         /* D3Q7 phase populations the next step reads */
+    void *f_next;     /* buffers the next step writes */
+    float *g_next;
+    uint64_t *step_counter; /* device word: inlet-noise step index for graph replays */
+    double *reduce;   /* SSMD_REDUCE_DOUBLES doubles of reduction workspace */
+    uint64_t owned;   /* nonzero when ssmd_alloc owns the buffers */
+} ssmd_fields;
+
+#define SSMD_REDUCE_DOUBLES 8192
+
+typedef struct ssmd_graph ssmd_graph;
+
+/* Library / build identification. */
+int ssmd_abi_version(void);
+const char *ssmd_error_string(int code);
+
+/* Fill *p with the reference's compile-time case (constants.cuh:70-101,
+ * globalFunctions.cuh:170-283) for lattice q: 128x256x256, water jet on y=0,
+ * oil jet on z=0, outflow on y and z, reference periodicX. */
+int ssmd_params_reference_default(ssmd_params *p, int32_t q);
+
+/* 0 if *p describes a case this build runs, else SSMD_EINVAL/SSMD_EUNSUPPORTED. */
+int ssmd_validate(const ssmd_params *p);
+
+/* Bytes of one buffer: which = 0 scalar field, 1 f, 2 g. 0 on bad params. */
+size_t ssmd_bytes(const ssmd_params *p, int32_t which);
+
+/* Allocate and zero every buffer (FieldAllocate semantics, hipMalloc). */
+int ssmd_alloc(const ssmd_params *p, ssmd_fields *d);
+/* Free buffers allocated by ssmd_alloc; no-op for caller-owned buffers. */
+int ssmd_free(ssmd_fields *d);
+
+/* t = 0 state: setWaterJet, setOilJet, setInitialDensity, setDistros
+ * (initialConditions.cu:33-140), written into both population buffers. */
+int ssmd_init(const ssmd_params *p, ssmd_fields *d, void *stream);
+
+/* One timestep: phase::computePhase, then one fused kernel for
+ * phase::computeForces + lbm::computeMoments + lbm::streamCollide, then the
+ * boundary kernels in reference order (main.cu:162-169). `step` indexes the
+ * inlet noise. Swaps f<->f_next and g<->g_next in *d. */
+int ssmd_step(const ssmd_params *p, ssmd_fields *d, uint64_t step, uint32_t flags, void *stream);
+
+/* The three launches ssmd_step makes, for callers that time or interleave
+ * them (the reference's separate kernels, LBMIncludes.cuh:29-47):
+ *   ssmd_launch_phase       phase::computePhase (reads d->g, writes phi)
+ *   ssmd_launch_lattice     computeForces + computeMoments + streamCollide
+ *                           (reads d->f, phi; writes d->f_next, d->g_next)
+ *   ssmd_launch_boundaries  K5-K9 and the y/z periodic copies on d->f_next,
+ *                           d->g_next (device step counter used if counter != 0)
+ *   ssmd_swap               host-side exchange f<->f_next, g<->g_next */
+int ssmd_launch_phase(const ssmd_params *p, const ssmd_fields *d, void *stream);
+int ssmd_launch_lattice(const ssmd_params *p, const ssmd_fields *d, uint32_t flags, void *stream);
+int ssmd_launch_boundaries(const ssmd_params *p, const ssmd_fields *d, uint64_t step, int32_t use_counter,
+                           void *stream);
+int ssmd_swap(ssmd_fields *d);
+
+/* nsteps calls of ssmd_step for steps step0 .. step0+nsteps-1. */
+int ssmd_run(const ssmd_params *p, ssmd_fields *d, uint64_t step0, uint64_t nsteps, void *stream);
+
+/* Capture the step (both buffer parities) into HIP graphs.  The boundary
+ * kernels are inside the graph; the inlet-noise step index comes from
+ * d->step_counter, advanced on the device by the last kernel of each step. */
+int ssmd_graph_create(const ssmd_params *p, const ssmd_fields *d, void *stream, ssmd_graph **out);
+/* Replay nsteps steps starting at step index step0; swaps *d as ssmd_step does. */
+int ssmd_graph_launch(ssmd_graph *g, ssmd_fields *d, uint64_t step0, uint64_t nsteps, void *stream);
+int ssmd_graph_destroy(ssmd_graph *g);
+
+/* FP64 reductions over interior cells, copied to out_host[4] (synchronizes
+ * the stream): sum(phi), sum(rho*ux), sum(rho*uy), sum(rho*uz).  The
+ * momentum entries need rho/u from a step run with SSMD_STEP_MACROS. */
+int ssmd_phase_mass_f64(const ssmd_params *p, const ssmd_fields *d, double *out_host, void *stream);
+
+/* Bit-exact storage conversions used by the kernels, evaluated on the device
+ * (n elements, device pointers): float32 -> format -> float32. */
+int ssmd_quantize(int32_t pop_format, const float *in, float *out, uint64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSMD_H */

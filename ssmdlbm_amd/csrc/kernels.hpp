@@ -1,0 +1,579 @@
+// kernels.hpp -- gfx950 kernels of the per-timestep path.
+//
+// Data layout in HBM (one field set):
+//   scalar fields  float[N], idx = x + nx*(y + ny*z)
+//   f, f_next      Q planes of N pop elements (f32 / f16 / bf16 deviations)
+//   g, g_next      7 planes of N floats
+// Populations are source-indexed ("pull"): slot [i][s] holds what leaves
+// cell s along c_i; the population arriving at cell a along c_i is slot
+// [i][a - c_i].  Every element is read once and written once per step.
+//
+// Per step:
+//   k_phase    phi(x) = sum_k g[k][x - c_k] on interior cells         (K1)
+//   k_lattice  fused phase::computeForces + lbm::computeMoments +
+//              lbm::streamCollide: LDS-staged phi tile with a radius-2
+//              halo marched along z, masked normals -> curvature -> F,
+//              regularized collision, f and g written at the own cell  (K2-K4)
+//   boundary kernels in reference order (K5-K9, + periodic y / z)
+#pragma once
+
+#include "lattice.hpp"
+
+namespace ssmd {
+
+struct JetK {
+    int enabled;
+    float ca, cb, r2, u, phi;
+};
+
+struct KParams {
+    int nx, ny, nz;
+    uint32_t sxy;  // nx*ny
+    uint32_t N;    // cells
+    float visc_water, visc_diff;
+    float omco_water, omco_oil, omco_ref;
+    float sigma, gamma;
+    float perturb;
+    uint64_t seed;
+    JetK water, oil;
+};
+
+struct Macro {
+    float *rho, *ux, *uy, *uz, *pxx, *pyy, *pzz, *pxy, *pxz, *pyz, *fsx, *fsy, *fsz;
+};
+
+__device__ __forceinline__ uint32_t cell_id(const KParams &k, int x, int y, int z) {
+    return uint32_t(x) + uint32_t(k.nx) * (uint32_t(y) + uint32_t(k.ny) * uint32_t(z));
+}
+
+__device__ __forceinline__ bool is_interior(const KParams &k, int x, int y, int z) {
+    return x >= 1 && x <= k.nx - 2 && y >= 1 && y <= k.ny - 2 && z >= 1 && z <= k.nz - 2;
+}
+
+// --------------------------------------------------------------------------
+// K1: phi = sum of the seven arriving phase populations (phaseField.cuh:31-52).
+__global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi) {
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= k.N) return;
+    const int x = int(c % uint32_t(k.nx));
+    const uint32_t r = c / uint32_t(k.nx);
+    const int y = int(r % uint32_t(k.ny));
+    const int z = int(r / uint32_t(k.ny));
+    if (!is_interior(k, x, y, z)) return;
+    const size_t N = k.N;
+    float s = g[c];
+    s += g[N + c - 1u];
+    s += g[2 * N + c + 1u];
+    s += g[3 * N + c - uint32_t(k.nx)];
+    s += g[4 * N + c + uint32_t(k.nx)];
+    s += g[5 * N + c - k.sxy];
+    s += g[6 * N + c + k.sxy];
+    phi[c] = s;
+}
+
+// --------------------------------------------------------------------------
+// Fused lattice kernel.  Block = TX x TY threads owning an x-y tile, marching
+// over LZ planes of z.  LDS rings (5 phi planes with a 2-cell halo, masked and
+// raw; 3 planes of masked normals with a 1-cell halo) give every thread the
+// radius-2 neighbourhood the force and normal stencils need.
+template <int Q, class P, int TX, int TY, bool MACROS>
+__global__ void __launch_bounds__(TX *TY)
+    k_lattice(KParams k, const typename P::T *__restrict__ fin, typename P::T *__restrict__ fout,
+              float *__restrict__ gout, const float *__restrict__ phi, Macro mac, int lz) {
+    constexpr int HX = TX + 4, HY = TY + 4, HP = HX * HY;
+    constexpr int NXl = TX + 2, NYl = TY + 2, NP = NXl * NYl;
+    constexpr int NT = TX * TY;
+    __shared__ float s_m[5][HP];   // phi-hat: phi on interior cells, 0 on ghosts / outside
+    __shared__ float s_r[5][HP];   // phi as stored (0 outside the domain)
+    __shared__ float4 s_n[3][NP];  // masked normal (x,y,z) and |grad phi-hat|
+
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int zb = 1 + int(blockIdx.z) * lz;
+    const int ze = min(zb + lz, k.nz - 1);
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x >= 1 && x <= k.nx - 2 && y >= 1 && y <= k.ny - 2;
+    const size_t N = k.N;
+
+    auto load_plane = [&](int p) {
+        const int sl = (p + 5) % 5;
+        const bool pin = p >= 0 && p < k.nz;
+        const bool pint = p >= 1 && p <= k.nz - 2;
+        for (int e = tid; e < HP; e += NT) {
+            const int gx = x0 + e % HX - 2, gy = y0 + e / HX - 2;
+            const bool in = pin && gx >= 0 && gx < k.nx && gy >= 0 && gy < k.ny;
+            const float v = in ? phi[cell_id(k, gx, gy, p)] : 0.f;
+            const bool ii = pint && gx >= 1 && gx <= k.nx - 2 && gy >= 1 && gy <= k.ny - 2;
+            s_r[sl][e] = v;
+            s_m[sl][e] = ii ? v : 0.f;
+        }
+    };
+    auto normals = [&](int p) {
+        const int sl = (p + 3) % 3;
+        const int pm = (p + 4) % 5, p0 = (p + 5) % 5, pp = (p + 6) % 5;
+        for (int e = tid; e < NP; e += NT) {
+            const int hx = e % NXl + 1, hy = e / NXl + 1;
+            auto at = [&](int dx, int dy, int dz) {
+                const int s = dz < 0 ? pm : (dz > 0 ? pp : p0);
+                return s_m[s][(hy + dy) * HX + hx + dx];
+            };
+            float gx, gy, gz;
+            iso_grad<Q>(at, gx, gy, gz);
+            const float ind = __fsqrt_rn(gx * gx + gy * gy + gz * gz);
+            const float inv = 1.0f / (ind + 1e-9f);
+            s_n[sl][e] = make_float4(gx * inv, gy * inv, gz * inv, ind);
+        }
+    };
+
+    for (int p = zb - 2; p <= zb + 1; ++p) load_plane(p);
+    __syncthreads();
+    normals(zb - 1);
+    normals(zb);
+
+    for (int z = zb; z < ze; ++z) {
+        const uint32_t c = cell_id(k, x, y, z);
+        float f[Q];
+        if (active) {
+            sfor<0, Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr Dir d = kDir[i];
+                const uint32_t src = c - uint32_t(d.x) - uint32_t(d.y) * uint32_t(k.nx) - uint32_t(d.z) * k.sxy;
+                f[i] = P::load(fin[size_t(i) * N + src]);
+            });
+        }
+        load_plane(z + 2);
+        __syncthreads();
+        normals(z + 1);
+        __syncthreads();
+        if (!active) continue;
+
+        // surface-tension force from the masked normals (phaseField.cuh:246-293)
+        const int nzm = (z + 2) % 3, nz0 = z % 3, nzp = (z + 1) % 3;
+        const int ncen = (ty + 1) * NXl + tx + 1;
+        float kx, ky, kz;
+        {
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            auto nat = [&](int dx, int dy, int dz) {
+                const int s = dz < 0 ? nzm : (dz > 0 ? nzp : nz0);
+                return s_n[s][ncen + dy * NXl + dx];
+            };
+            float acc[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+            sfor<0, (Q - 1) / 2>([&](auto Pp) {
+                constexpr int i = 2 * Pp + 1;
+                constexpr Dir d = kDir[i];
+                constexpr int cls = i <= 6 ? 0 : (i <= 18 ? 1 : 2);
+                const float4 a = nat(d.x, d.y, d.z), b = nat(-d.x, -d.y, -d.z);
+                if constexpr (d.x != 0) acc[cls][0] += d.x > 0 ? a.x - b.x : b.x - a.x;
+                if constexpr (d.y != 0) acc[cls][1] += d.y > 0 ? a.y - b.y : b.y - a.y;
+                if constexpr (d.z != 0) acc[cls][2] += d.z > 0 ? a.z - b.z : b.z - a.z;
+            });
+            constexpr float w1 = weight<Q>(1), w2 = weight<Q>(7), w3 = Q == 27 ? weight<Q>(19) : 0.f;
+            sx = w1 * acc[0][0] + w2 * acc[1][0];
+            sy = w1 * acc[0][1] + w2 * acc[1][1];
+            sz = w1 * acc[0][2] + w2 * acc[1][2];
+            if constexpr (Q == 27) {
+                sx += w3 * acc[2][0];
+                sy += w3 * acc[2][1];
+                sz += w3 * acc[2][2];
+            }
+            const float curvature = 3.0f * (sx + sy + sz);
+            const float4 nc = s_n[nz0][ncen];
+            const float st = -k.sigma * curvature * nc.w;
+            kx = st * nc.x;
+            ky = st * nc.y;
+            kz = st * nc.z;
+        }
+
+        // normals of the stored phi for the phase update (lbm.cu:157-224)
+        const int rzm = (z + 4) % 5, rz0 = z % 5, rzp = (z + 1) % 5;
+        const int rcen = (ty + 2) * HX + tx + 2;
+        float nx, ny, nz, ph;
+        {
+            auto at = [&](int dx, int dy, int dz) {
+                const int s = dz < 0 ? rzm : (dz > 0 ? rzp : rz0);
+                return s_r[s][rcen + dy * HX + dx];
+            };
+            float gx, gy, gz;
+            iso_grad<Q>(at, gx, gy, gz);
+            const float ind = __fsqrt_rn(gx * gx + gy * gy + gz * gz);
+            const float inv = 1.0f / (ind + 1e-9f);
+            nx = gx * inv;
+            ny = gy * inv;
+            nz = gz * inv;
+            ph = s_r[rz0][rcen];
+        }
+
+        const Cell m = moments<Q>(f, kx, ky, kz);
+        const float nu = __builtin_fmaf(ph, k.visc_diff, k.visc_water);
+        const float omega = 1.0f / (0.5f + 3.0f * nu);
+        const float omco = 1.0f - omega;
+        collide<Q>(m, omco, kx, ky, kz, [&](int i, float v) { fout[size_t(i) * N + c] = P::store(v); });
+        const float sharp = k.gamma * ph * (1.0f - ph);
+        phase_collide(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) { gout[size_t(kk) * N + c] = v; });
+
+        const bool face = x == 1 || x == k.nx - 2 || y == 1 || y == k.ny - 2 || z == 1 || z == k.nz - 2;
+        if (MACROS || face) {
+            mac.rho[c] = m.rho;
+            mac.ux[c] = m.ux;
+            mac.uy[c] = m.uy;
+            mac.uz[c] = m.uz;
+            mac.pxx[c] = m.pxx;
+            mac.pyy[c] = m.pyy;
+            mac.pzz[c] = m.pzz;
+            mac.pxy[c] = m.pxy;
+            mac.pxz[c] = m.pxz;
+            mac.pyz[c] = m.pyz;
+        }
+        if (MACROS) {
+            mac.fsx[c] = kx;
+            mac.fsy[c] = ky;
+            mac.fsz[c] = kz;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Boundary kernels.  Each writes the SOURCE slots of the populations the
+// reference writes at their arrival cells, so arrival a along c_i maps to
+// slot (a - c_i).
+
+// Seeded inlet noise: SplitMix64 chained over (seed, step, cell, component).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float noise_uniform(uint64_t seed, uint64_t step, uint64_t cell, uint64_t comp) {
+    const uint64_t h = splitmix64(splitmix64(splitmix64(seed ^ step) ^ cell) ^ comp);
+    return float(uint32_t(h >> 40)) * 1.1920928955078125e-7f - 1.0f;
+}
+
+// Non-equilibrium reconstruction at a fluid node (no force term,
+// BoundaryConditions.cuh:87-94): feq(rho, u_bc) + omco * fneq(Pi, u at node).
+template <int Q, int I>
+__device__ __forceinline__ float bc_population(const KParams &k, const Macro &mac, uint32_t fl, float rho, float ux,
+                                               float uy, float uz, float omco) {
+    constexpr Dir c = kDir[I];
+    constexpr float w = weight<Q>(I);
+    const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const float cu = 3.0f * (float(c.x) * ux + float(c.y) * uy + float(c.z) * uz);
+    float feq = rho * (1.0f - uu + cu + 0.5f * cu * cu);
+    if constexpr (Q == 27) feq += rho * ((1.0f / 6.0f) * cu * cu * cu - uu * cu);
+    feq = w * feq - w;
+    const float pxx = mac.pxx[fl], pyy = mac.pyy[fl], pzz = mac.pzz[fl];
+    const float pxy = mac.pxy[fl], pxz = mac.pxz[fl], pyz = mac.pyz[fl];
+    float s = (float(c.x * c.x) - kCs2) * pxx + (float(c.y * c.y) - kCs2) * pyy + (float(c.z * c.z) - kCs2) * pzz +
+              2.0f * (float(c.x * c.y) * pxy + float(c.x * c.z) * pxz + float(c.y * c.z) * pyz);
+    if constexpr (Q == 27) {
+        const float nux = mac.ux[fl], nuy = mac.uy[fl], nuz = mac.uz[fl];
+        constexpr int kxxy = 3 * c.x * c.x * c.y - c.y;
+        constexpr int kxxz = 3 * c.x * c.x * c.z - c.z;
+        constexpr int kxyy = 3 * c.x * c.y * c.y - c.x;
+        constexpr int kyyz = 3 * c.y * c.y * c.z - c.z;
+        constexpr int kxzz = 3 * c.x * c.z * c.z - c.x;
+        constexpr int kyzz = 3 * c.y * c.z * c.z - c.y;
+        constexpr int kxyz = 6 * c.x * c.y * c.z;
+        s += float(kxxy) * (pxx * nuy + 2.0f * nux * pxy) + float(kxxz) * (pxx * nuz + 2.0f * nux * pxz) +
+             float(kxyy) * (pyy * nux + 2.0f * nuy * pxy) + float(kyyz) * (pyy * nuz + 2.0f * nuy * pyz) +
+             float(kxzz) * (pzz * nux + 2.0f * nuz * pxz) + float(kyzz) * (pzz * nuy + 2.0f * nuz * pyz) +
+             float(kxyz) * (nux * pyz + nuy * pxz + nuz * pxy);
+    }
+    return feq + omco * ((w * 4.5f) * s);
+}
+
+// Device step index for graph replays: counter if given, else the argument.
+__device__ __forceinline__ uint64_t step_index(const uint64_t *counter, uint64_t step) {
+    return counter ? *counter : step;
+}
+__device__ __forceinline__ void bump(uint64_t *counter, int doit) {
+    if (doit && counter && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *counter += 1;
+}
+
+// applyWaterInflow (axis 1, face y=0) / applyOilInflow (axis 2, face z=0).
+template <int Q, class P, int AXIS>
+__global__ void k_inflow(KParams k, typename P::T *__restrict__ fout, float *__restrict__ gout, float *__restrict__ phi,
+                         Macro mac, const uint64_t *counter, uint64_t step) {
+    const JetK &j = AXIS == 1 ? k.water : k.oil;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nb = AXIS == 1 ? k.nz : k.ny;
+    if (a < k.nx && b < nb) {
+        const float da = float(a) - j.ca, db = float(b) - j.cb;
+        if (da * da + db * db <= j.r2) {
+            const int x = a, y = AXIS == 1 ? 0 : b, z = AXIS == 1 ? b : 0;
+            const uint32_t bnd = cell_id(k, x, y, z);
+            float u3[3] = {0.f, 0.f, 0.f};
+            if (k.perturb != 0.f) {
+                const uint64_t s = step_index(counter, step);
+                const float amp = k.perturb * j.u;
+                for (int q = 0; q < 3; ++q) u3[q] = amp * noise_uniform(k.seed, s, bnd, uint64_t(q));
+            }
+            u3[AXIS] += j.u;
+            const float ux = u3[0], uy = u3[1], uz = u3[2];
+            const float rho = 1.0f, ph = j.phi;
+            mac.rho[bnd] = rho;
+            phi[bnd] = ph;
+            mac.ux[bnd] = ux;
+            mac.uy[bnd] = uy;
+            mac.uz[bnd] = uz;
+            const float omco = AXIS == 1 ? k.omco_water : k.omco_oil;
+            sfor<0, Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr Dir c = kDir[i];
+                if constexpr ((AXIS == 1 ? c.y : c.z) == 1) {
+                    const int xx = x + c.x, yy = y + c.y, zz = z + c.z;
+                    bool ok;
+                    if constexpr (AXIS == 2)
+                        ok = xx >= 1 && xx <= k.nx - 2 && yy >= 1 && yy <= k.ny - 2;
+                    else
+                        ok = xx >= 0 && xx < k.nx && zz >= 0 && zz < k.nz;
+                    if (ok) {
+                        const uint32_t fl = cell_id(k, xx, yy, zz);
+                        fout[size_t(i) * k.N + bnd] = P::store(bc_population<Q, i>(k, mac, fl, rho, ux, uy, uz, omco));
+                    }
+                }
+            });
+            const int kk = AXIS == 1 ? 3 : 5;
+            const float un = AXIS == 1 ? uy : uz;
+            gout[size_t(kk) * k.N + bnd] = weight7(kk) * ph * (1.0f + 4.0f * un);
+        }
+    }
+}
+
+// applyOutflowY (axis 1) / applyOutflowZ (axis 2).
+template <int Q, class P, int AXIS>
+__global__ void k_outflow(KParams k, typename P::T *__restrict__ fout, float *__restrict__ gout, float *__restrict__ phi,
+                          Macro mac, uint64_t *counter, int bump_last) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nb = AXIS == 1 ? k.nz : k.ny;
+    if (a >= 1 && a <= k.nx - 2 && b >= 1 && b <= nb - 2) {
+        const int x = a, y = AXIS == 1 ? k.ny - 1 : b, z = AXIS == 1 ? b : k.nz - 1;
+        const uint32_t bnd = cell_id(k, x, y, z);
+        const uint32_t in = AXIS == 1 ? cell_id(k, x, k.ny - 2, z) : cell_id(k, x, y, k.nz - 2);
+        const float rho = mac.rho[in], ph = phi[in], ux = mac.ux[in], uy = mac.uy[in], uz = mac.uz[in];
+        mac.rho[bnd] = rho;
+        phi[bnd] = ph;
+        mac.ux[bnd] = ux;
+        mac.uy[bnd] = uy;
+        mac.uz[bnd] = uz;
+        sfor<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr Dir c = kDir[i];
+            if constexpr ((AXIS == 1 ? c.y : c.z) == -1) {
+                const int xx = x + c.x;
+                const int yy = AXIS == 1 ? k.ny - 2 : y + c.y;
+                const int zz = AXIS == 1 ? z + c.z : k.nz - 2;
+                bool ok = true;
+                if constexpr (AXIS == 1) ok = xx >= 1 && xx <= k.nx - 2 && zz >= 1 && zz <= k.nz - 2;
+                if (ok) {
+                    const uint32_t fl = cell_id(k, xx, yy, zz);
+                    fout[size_t(i) * k.N + bnd] = P::store(bc_population<Q, i>(k, mac, fl, rho, ux, uy, uz, k.omco_ref));
+                }
+            }
+        });
+        const int kk = AXIS == 1 ? 4 : 6;
+        const float un = AXIS == 1 ? uy : uz;
+        gout[size_t(kk) * k.N + bnd] = weight7(kk) * ph * (1.0f - 4.0f * un);
+    }
+    bump(counter, bump_last);
+}
+
+// periodicX (BoundaryConditions.cuh:295-349) and its mirrors on y and z.
+// Face coordinates (a, b) run over the two other axes in increasing order.
+template <int Q, class P, int AXIS, int MODE>
+__global__ void k_periodic(KParams k, typename P::T *__restrict__ fout, float *__restrict__ gout,
+                           float *__restrict__ phi, Macro mac, uint64_t *counter, int bump_last) {
+    constexpr int A1 = AXIS == 0 ? 1 : 0, A2 = AXIS == 2 ? 1 : 2;
+    const int n[3] = {k.nx, k.ny, k.nz};
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    if (a < n[A1] && b < n[A2]) {
+        const int nn = n[AXIS];
+        auto at = [&](int s, int aa, int bb) {
+            int r[3];
+            r[AXIS] = s;
+            r[A1] = aa;
+            r[A2] = bb;
+            return cell_id(k, r[0], r[1], r[2]);
+        };
+        sfor<1, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr Dir c = kDir[i];
+            constexpr int cv[3] = {c.x, c.y, c.z};
+            constexpr int ca = cv[AXIS], c1 = cv[A1], c2 = cv[A2];
+            if constexpr (ca != 0) {
+                if constexpr (MODE == 1) {
+                    // arrival planes 1 <-> n-2 exchange, expressed on source slots
+                    const int sa = a - c1, sb = b - c2;
+                    if (sa >= 0 && sa < n[A1] && sb >= 0 && sb < n[A2]) {
+                        const uint32_t dst = ca > 0 ? at(0, sa, sb) : at(nn - 1, sa, sb);
+                        const uint32_t src = ca > 0 ? at(nn - 3, sa, sb) : at(2, sa, sb);
+                        fout[size_t(i) * k.N + dst] = fout[size_t(i) * k.N + src];
+                    }
+                } else {
+                    const uint32_t dst = ca > 0 ? at(0, a, b) : at(nn - 1, a, b);
+                    const uint32_t src = ca > 0 ? at(nn - 2, a, b) : at(1, a, b);
+                    fout[size_t(i) * k.N + dst] = fout[size_t(i) * k.N + src];
+                }
+            }
+        });
+        // D3Q7: the two directions along AXIS
+        {
+            const int kp = 1 + 2 * AXIS, km = 2 + 2 * AXIS;
+            if (MODE == 1) {
+                gout[size_t(kp) * k.N + at(0, a, b)] = gout[size_t(kp) * k.N + at(nn - 3, a, b)];
+                gout[size_t(km) * k.N + at(nn - 1, a, b)] = gout[size_t(km) * k.N + at(2, a, b)];
+            } else {
+                gout[size_t(kp) * k.N + at(0, a, b)] = gout[size_t(kp) * k.N + at(nn - 2, a, b)];
+                gout[size_t(km) * k.N + at(nn - 1, a, b)] = gout[size_t(km) * k.N + at(1, a, b)];
+            }
+        }
+        const uint32_t bL = at(1, a, b), bR = at(nn - 2, a, b), gL = at(0, a, b), gR = at(nn - 1, a, b);
+        phi[gL] = phi[bR];
+        phi[gR] = phi[bL];
+        mac.ux[gL] = mac.ux[bR];
+        mac.ux[gR] = mac.ux[bL];
+        mac.uy[gL] = mac.uy[bR];
+        mac.uy[gR] = mac.uy[bL];
+        mac.uz[gL] = mac.uz[bR];
+        mac.uz[gR] = mac.uz[bL];
+    }
+    bump(counter, bump_last);
+}
+
+__global__ void k_set_counter(uint64_t *counter, uint64_t v) { *counter = v; }
+// Advances the device step index when no outflow/periodic kernel follows
+// the inlets (the inlet kernels read the counter, so they never write it).
+__global__ void k_bump(uint64_t *counter) { *counter += 1; }
+
+// --------------------------------------------------------------------------
+// t = 0 state (initialConditions.cu:33-140) in the source-indexed layout:
+// slot [i][s] = reference f[i][s + c_i] = feq_i(rho, u at s + c_i).
+__device__ __forceinline__ void init_macros(const KParams &k, int x, int y, int z, float &phi, float &ux, float &uy,
+                                            float &uz) {
+    phi = 0.f;
+    ux = uy = uz = 0.f;
+    if (k.water.enabled && y == 0) {
+        const float da = float(x) - k.water.ca, db = float(z) - k.water.cb;
+        if (da * da + db * db <= k.water.r2) {
+            phi = k.water.phi;
+            uy = k.water.u;
+        }
+    }
+    if (k.oil.enabled && z == 0) {
+        const float da = float(x) - k.oil.ca, db = float(y) - k.oil.cb;
+        if (da * da + db * db <= k.oil.r2) {
+            phi = k.oil.phi;
+            uz = k.oil.u;
+        }
+    }
+}
+
+template <int Q, class P>
+__global__ void __launch_bounds__(256) k_init(KParams k, typename P::T *f0, typename P::T *f1, float *g0, float *g1,
+                                              float *phi, Macro mac) {
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= k.N) return;
+    const int x = int(c % uint32_t(k.nx));
+    const uint32_t r = c / uint32_t(k.nx);
+    const int y = int(r % uint32_t(k.ny));
+    const int z = int(r / uint32_t(k.ny));
+    float ph, ux, uy, uz;
+    init_macros(k, x, y, z, ph, ux, uy, uz);
+    phi[c] = ph;
+    mac.rho[c] = 1.0f;
+    mac.ux[c] = ux;
+    mac.uy[c] = uy;
+    mac.uz[c] = uz;
+    mac.pxx[c] = mac.pyy[c] = mac.pzz[c] = mac.pxy[c] = mac.pxz[c] = mac.pyz[c] = 0.f;
+    mac.fsx[c] = mac.fsy[c] = mac.fsz[c] = 0.f;
+    sfor<0, Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr Dir d = kDir[i];
+        constexpr float w = weight<Q>(i);
+        const int ax = x + d.x, ay = y + d.y, az = z + d.z;
+        float v = 0.f;
+        if (ax >= 0 && ax < k.nx && ay >= 0 && ay < k.ny && az >= 0 && az < k.nz) {
+            float aph, aux, auy, auz;
+            init_macros(k, ax, ay, az, aph, aux, auy, auz);
+            const float uu = 1.5f * (aux * aux + auy * auy + auz * auz);
+            const float cu = 3.0f * (float(d.x) * aux + float(d.y) * auy + float(d.z) * auz);
+            float e = 1.0f - uu + cu + 0.5f * cu * cu;
+            if constexpr (Q == 27) e += (1.0f / 6.0f) * cu * cu * cu - uu * cu;
+            v = w * e - w;
+        }
+        const typename P::T sv = P::store(v);
+        f0[size_t(i) * k.N + c] = sv;
+        f1[size_t(i) * k.N + c] = sv;
+    });
+    sfor<0, 7>([&](auto K) {
+        constexpr int kk = K;
+        constexpr Dir d = kDir[kk];
+        const int ax = x + d.x, ay = y + d.y, az = z + d.z;
+        float v = 0.f;
+        if (ax >= 0 && ax < k.nx && ay >= 0 && ay < k.ny && az >= 0 && az < k.nz) {
+            float aph, aux, auy, auz;
+            init_macros(k, ax, ay, az, aph, aux, auy, auz);
+            v = weight7(kk) * aph * (1.0f + 4.0f * (float(d.x) * aux + float(d.y) * auy + float(d.z) * auz));
+        }
+        g0[size_t(kk) * k.N + c] = v;
+        g1[size_t(kk) * k.N + c] = v;
+    });
+}
+
+// --------------------------------------------------------------------------
+// FP64 reductions over interior cells: sum phi, rho*ux, rho*uy, rho*uz.
+// Pass 1 writes one partial per block; pass 2 (one block) sums them in a
+// fixed order, so the result is deterministic.
+__global__ void __launch_bounds__(256) k_reduce1(KParams k, const float *phi, Macro mac, double *part) {
+    __shared__ double sh[4][256];
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < k.N; c += gridDim.x * 256u) {
+        const int x = int(c % uint32_t(k.nx));
+        const uint32_t r = c / uint32_t(k.nx);
+        const int y = int(r % uint32_t(k.ny));
+        const int z = int(r / uint32_t(k.ny));
+        if (!is_interior(k, x, y, z)) continue;
+        const double rho = mac.rho[c];
+        s0 += double(phi[c]);
+        s1 += rho * double(mac.ux[c]);
+        s2 += rho * double(mac.uy[c]);
+        s3 += rho * double(mac.uz[c]);
+    }
+    sh[0][threadIdx.x] = s0;
+    sh[1][threadIdx.x] = s1;
+    sh[2][threadIdx.x] = s2;
+    sh[3][threadIdx.x] = s3;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w)
+            for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_reduce2(const double *part, int nparts, double *out) {
+    __shared__ double sh[4][256];
+    double s[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nparts; b += 256)
+        for (int q = 0; q < 4; ++q) s[q] += part[b * 4 + q];
+    for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w)
+            for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <class P>
+__global__ void k_quantize(const float *in, float *out, uint64_t n) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = P::load(P::store(in[i]));
+}
+
+}  // namespace ssmd

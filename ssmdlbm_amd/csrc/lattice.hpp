@@ -1,0 +1,249 @@
+// lattice.hpp -- compile-time velocity sets and per-cell collision algebra
+// for the gfx950 kernels.
+//
+// Direction order and weights are those of the reference velocity sets
+// (src/velocitySet/D3Q19.cuh:50-131, D3Q27.cuh:233-323, D3Q7.cuh:440-512):
+// the boundary kernels address populations by number and opposite
+// directions are the pairs (2k-1, 2k), which the collision below exploits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+namespace ssmd {
+
+template <int B, int E, class F>
+__device__ __forceinline__ void sfor(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        sfor<B + 1, E>(f);
+    }
+}
+
+struct Dir {
+    int x, y, z;
+};
+
+// Shared table: D3Q19 is the first 19 entries, D3Q7 the first 7.
+constexpr Dir kDir[27] = {
+    {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
+    {1, 1, 0},   {-1, -1, 0}, {1, 0, 1},   {-1, 0, -1}, {0, 1, 1},   {0, -1, -1}, {1, -1, 0},
+    {-1, 1, 0},  {1, 0, -1},  {-1, 0, 1},  {0, 1, -1},  {0, -1, 1},  {1, 1, 1},   {-1, -1, -1},
+    {1, 1, -1},  {-1, -1, 1}, {1, -1, 1},  {-1, 1, -1}, {-1, 1, 1},  {1, -1, -1}};
+
+template <int Q>
+constexpr float weight(int i) {
+    if constexpr (Q == 19) {
+        return i == 0 ? float(1.0 / 3.0) : (i <= 6 ? float(1.0 / 18.0) : float(1.0 / 36.0));
+    } else {
+        return i == 0 ? float(8.0 / 27.0)
+                      : (i <= 6 ? float(2.0 / 27.0) : (i <= 18 ? float(1.0 / 54.0) : float(1.0 / 216.0)));
+    }
+}
+constexpr float weight7(int k) { return k == 0 ? 0.25f : 0.125f; }
+
+constexpr float kCs2 = float(1.0 / 3.0);
+
+// --------------------------------------------------------------------------
+// Population storage formats (deviation fbar = f - w is what is stored).
+struct PopF32 {
+    using T = float;
+    static __device__ __forceinline__ float load(T v) { return v; }
+    static __device__ __forceinline__ T store(float v) { return v; }
+};
+struct PopF16 {
+    using T = uint16_t;
+    static __device__ __forceinline__ float load(T v) { return float(__builtin_bit_cast(_Float16, v)); }
+    // IEEE conversion in the default round-to-nearest-even mode.
+    static __device__ __forceinline__ T store(float v) { return __builtin_bit_cast(uint16_t, _Float16(v)); }
+};
+struct PopBF16 {
+    using T = uint16_t;
+    static __device__ __forceinline__ float load(T v) { return __builtin_bit_cast(float, uint32_t(v) << 16); }
+    static __device__ __forceinline__ T store(float v) {
+        uint32_t x = __builtin_bit_cast(uint32_t, v);
+        if ((x & 0x7FFFFFFFu) > 0x7F800000u) return uint16_t((x >> 16) | 0x40u);
+        x += 0x7FFFu + ((x >> 16) & 1u);
+        return uint16_t(x >> 16);
+    }
+};
+
+// --------------------------------------------------------------------------
+// Isotropic gradient 3 * sum_i w_i c_i v(x + c_i) over opposite pairs,
+// grouped by weight class (reference phaseField.cuh:158-218, lbm.cu:157-217).
+// `at(dx,dy,dz)` returns the stencil value.
+template <int Q, class At>
+__device__ __forceinline__ void iso_grad(At &&at, float &gx, float &gy, float &gz) {
+    float s[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};  // [class][axis]
+    sfor<0, (Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * P + 1;
+        constexpr Dir c = kDir[i];
+        constexpr int cls = i <= 6 ? 0 : (i <= 18 ? 1 : 2);
+        const float d = at(c.x, c.y, c.z) - at(-c.x, -c.y, -c.z);
+        if constexpr (c.x != 0) s[cls][0] += c.x > 0 ? d : -d;
+        if constexpr (c.y != 0) s[cls][1] += c.y > 0 ? d : -d;
+        if constexpr (c.z != 0) s[cls][2] += c.z > 0 ? d : -d;
+    });
+    constexpr float w1 = weight<Q>(1), w2 = weight<Q>(7), w3 = Q == 27 ? weight<Q>(19) : 0.f;
+    float r[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float v = w1 * s[0][a] + w2 * s[1][a];
+        if constexpr (Q == 27) v += w3 * s[2][a];
+        r[a] = 3.0f * v;
+    }
+    gx = r[0];
+    gy = r[1];
+    gz = r[2];
+}
+
+// --------------------------------------------------------------------------
+// Regularized collision of one cell (reference lbm.cu:33-128 moments,
+// lbm.cu:226-269 collision; D3Q19.cuh / D3Q27.cuh equilibrium, Hermite
+// non-equilibrium and Guo force).
+//
+// Moments use closed forms of the reference sums, exact in real arithmetic
+// for both lattices (they carry 4th-order isotropy):
+//   sum_i c_a c_b fbar_eq_i = rho u_a u_b + cs2 (rho-1) delta_ab
+//   sum_i c_a c_b F_i       = u_a F_b + u_b F_a
+// so Pi = traceless( S - rho u u + (u F + F u)/2 ),  S = sum_i c c fbar_i.
+// Post-collision populations are split into even and odd parts in c and
+// emitted in opposite pairs.
+struct Cell {
+    float rho, drho, ux, uy, uz;
+    float pxx, pyy, pzz, pxy, pxz, pyz;
+};
+
+template <int Q>
+__device__ __forceinline__ Cell moments(const float (&f)[Q], float fx, float fy, float fz) {
+    Cell m;
+    float dr = f[0];
+    float jx = 0.f, jy = 0.f, jz = 0.f;
+    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+    sfor<0, (Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * P + 1;
+        constexpr Dir c = kDir[i];
+        const float a = f[i], b = f[i + 1];
+        const float sum = a + b, dif = a - b;
+        dr += sum;
+        if constexpr (c.x != 0) { jx += c.x > 0 ? dif : -dif; sxx += sum; }
+        if constexpr (c.y != 0) { jy += c.y > 0 ? dif : -dif; syy += sum; }
+        if constexpr (c.z != 0) { jz += c.z > 0 ? dif : -dif; szz += sum; }
+        if constexpr (c.x * c.y != 0) sxy += c.x * c.y > 0 ? sum : -sum;
+        if constexpr (c.x * c.z != 0) sxz += c.x * c.z > 0 ? sum : -sum;
+        if constexpr (c.y * c.z != 0) syz += c.y * c.z > 0 ? sum : -sum;
+    });
+    const float rho = dr + 1.0f;
+    const float inv = 1.0f / rho;
+    const float ux = inv * jx + fx * 0.5f * inv;
+    const float uy = inv * jy + fy * 0.5f * inv;
+    const float uz = inv * jz + fz * 0.5f * inv;
+    float pxx = sxx - rho * ux * ux + ux * fx;
+    float pyy = syy - rho * uy * uy + uy * fy;
+    float pzz = szz - rho * uz * uz + uz * fz;
+    const float tr = pxx + pyy + pzz;
+    pxx -= kCs2 * tr;
+    pyy -= kCs2 * tr;
+    pzz -= kCs2 * tr;
+    m.rho = rho;
+    m.drho = dr;
+    m.ux = ux;
+    m.uy = uy;
+    m.uz = uz;
+    m.pxx = pxx;
+    m.pyy = pyy;
+    m.pzz = pzz;
+    m.pxy = sxy - rho * ux * uy + 0.5f * (ux * fy + uy * fx);
+    m.pxz = sxz - rho * ux * uz + 0.5f * (ux * fz + uz * fx);
+    m.pyz = syz - rho * uy * uz + 0.5f * (uy * fz + uz * fy);
+    return m;
+}
+
+// Emits w_i * (even +- odd) for every direction through `emit(i, value)`.
+// omco = 1 - omega.  With omco and F the reference's values this is
+// feq + omco*fneq + F/2; a boundary reconstruction without force passes F=0.
+template <int Q, class Emit>
+__device__ __forceinline__ void collide(const Cell &m, float omco, float fx, float fy, float fz, Emit &&emit) {
+    const float ux = m.ux, uy = m.uy, uz = m.uz;
+    const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const float uF = ux * fx + uy * fy + uz * fz;
+    const float t = kCs2 * (m.pxx + m.pyy + m.pzz);
+    const float o45 = 4.5f * omco;
+    // third-order coefficient combinations (D3Q27.cuh:358-367)
+    float axxy = 0.f, axxz = 0.f, axyy = 0.f, ayyz = 0.f, axzz = 0.f, ayzz = 0.f, axyz = 0.f;
+    if constexpr (Q == 27) {
+        axxy = m.pxx * uy + 2.0f * ux * m.pxy;
+        axxz = m.pxx * uz + 2.0f * ux * m.pxz;
+        axyy = m.pyy * ux + 2.0f * uy * m.pxy;
+        ayyz = m.pyy * uz + 2.0f * uy * m.pyz;
+        axzz = m.pzz * ux + 2.0f * uz * m.pxz;
+        ayzz = m.pzz * uy + 2.0f * uz * m.pyz;
+        axyz = ux * m.pyz + uy * m.pxz + uz * m.pxy;
+    }
+    // rest population
+    {
+        constexpr float w0 = weight<Q>(0);
+        const float e = m.drho * (1.0f - uu) - uu - o45 * t - 1.5f * uF;
+        emit(0, w0 * e);
+    }
+    sfor<0, (Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * P + 1;
+        constexpr Dir c = kDir[i];
+        constexpr float w = weight<Q>(i);
+        float cu = 0.f, cF = 0.f, h2 = -t;
+        if constexpr (c.x != 0) { cu += c.x > 0 ? ux : -ux; cF += c.x > 0 ? fx : -fx; h2 += m.pxx; }
+        if constexpr (c.y != 0) { cu += c.y > 0 ? uy : -uy; cF += c.y > 0 ? fy : -fy; h2 += m.pyy; }
+        if constexpr (c.z != 0) { cu += c.z > 0 ? uz : -uz; cF += c.z > 0 ? fz : -fz; h2 += m.pzz; }
+        if constexpr (c.x * c.y != 0) h2 += c.x * c.y > 0 ? 2.0f * m.pxy : -2.0f * m.pxy;
+        if constexpr (c.x * c.z != 0) h2 += c.x * c.z > 0 ? 2.0f * m.pxz : -2.0f * m.pxz;
+        if constexpr (c.y * c.z != 0) h2 += c.y * c.z > 0 ? 2.0f * m.pyz : -2.0f * m.pyz;
+        cu *= 3.0f;
+        const float hc = 0.5f * cu * cu - uu;  // X - 1 with X = 1 - uu + cu^2/2
+        const float even = m.drho * (1.0f + hc) + hc + o45 * h2 + 1.5f * (cu * cF - uF);
+        float odd_u = 1.0f;
+        if constexpr (Q == 27) odd_u += cu * cu * (1.0f / 6.0f) - uu;
+        float odd = m.rho * cu * odd_u + 1.5f * cF;
+        if constexpr (Q == 27) {
+            // 3 * (c_a c_a c_b - cs2 c_b) and 6 c_x c_y c_z, all integers
+            constexpr int kxxy = 3 * c.x * c.x * c.y - c.y;
+            constexpr int kxxz = 3 * c.x * c.x * c.z - c.z;
+            constexpr int kxyy = 3 * c.x * c.y * c.y - c.x;
+            constexpr int kyyz = 3 * c.y * c.y * c.z - c.z;
+            constexpr int kxzz = 3 * c.x * c.z * c.z - c.x;
+            constexpr int kyzz = 3 * c.y * c.z * c.z - c.y;
+            constexpr int kxyz = 6 * c.x * c.y * c.z;
+            float h3 = 0.f;
+            if constexpr (kxxy != 0) h3 += float(kxxy) * axxy;
+            if constexpr (kxxz != 0) h3 += float(kxxz) * axxz;
+            if constexpr (kxyy != 0) h3 += float(kxyy) * axyy;
+            if constexpr (kyyz != 0) h3 += float(kyyz) * ayyz;
+            if constexpr (kxzz != 0) h3 += float(kxzz) * axzz;
+            if constexpr (kyzz != 0) h3 += float(kyzz) * ayzz;
+            if constexpr (kxyz != 0) h3 += float(kxyz) * axyz;
+            odd += o45 * h3;
+        }
+        emit(i, w * (even + odd));
+        emit(i + 1, w * (even - odd));
+    });
+}
+
+// D3Q7 post-collision phase population (D3Q7.cuh:514-538, lbm.cu:256-269):
+// g_k = w_k phi (1 + 4 c_k.u) + w_k gamma phi (1-phi) c_k.n
+template <class Emit>
+__device__ __forceinline__ void phase_collide(float phi, float sharp, float ux, float uy, float uz, float nx,
+                                              float ny, float nz, Emit &&emit) {
+    emit(0, weight7(0) * phi);
+    const float u[3] = {ux, uy, uz};
+    const float n[3] = {nx, ny, nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float e = phi;
+        const float o = 4.0f * phi * u[a] + sharp * n[a];
+        emit(1 + 2 * a, weight7(1) * (e + o));
+        emit(2 + 2 * a, weight7(1) * (e - o));
+    }
+}
+
+}  // namespace ssmd

@@ -1,0 +1,194 @@
+"""Host-side driver of the gfx950 hot path, mirroring the reference's
+main.cu flow (allocate -> init -> per-step graph + boundaries -> output).
+
+Device memory comes from torch (so fields are tensors that tests and tools
+can read); every kernel runs inside libssmd.so through the C ABI on the
+torch stream of the device.  There is no CPU or torch compute path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Iterable, Optional
+
+import numpy as np
+import torch
+
+from . import abi
+from .cases import DIRECTIONS
+
+_POP_DTYPE = {abi.POP_F32: torch.float32, abi.POP_F16: torch.float16, abi.POP_BF16: torch.bfloat16}
+
+
+class Simulation:
+    """One field set on one GPU (reference: the global LBMFields + FieldAllocate)."""
+
+    def __init__(self, params: abi.Params, device: str | torch.device = "cuda:0"):
+        self.lib = abi.load()
+        abi.check(self.lib.ssmd_validate(C.byref(params)), "ssmd_validate")
+        if not torch.cuda.is_available():
+            raise abi.SsmdError("no GPU: the hot path runs only on the device")
+        self.p = params.copy()
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        # a stream of our own (default priority): graph capture cannot use the null stream
+        self.stream = torch.cuda.Stream(self.device)
+        n = self.p.cells
+        q = int(self.p.q)
+        z = lambda *shape, dtype=torch.float32: torch.zeros(*shape, dtype=dtype, device=self.device)
+        self.t: Dict[str, torch.Tensor] = {}
+        for name in abi.SCALAR_FIELDS:
+            if name in ("normx", "normy", "normz", "ind"):
+                continue  # never allocated, as in the reference (main.cu:55-70)
+            self.t[name] = z(n)
+        pdt = _POP_DTYPE[int(self.p.pop_format)]
+        self._pops = {"fa": z(q * n, dtype=pdt), "fb": z(q * n, dtype=pdt), "ga": z(7 * n), "gb": z(7 * n)}
+        self.t["step_counter"] = z(1, dtype=torch.int64)
+        self.t["reduce"] = z(abi.REDUCE_DOUBLES, dtype=torch.float64)
+        self.d = abi.Fields()
+        for name, ten in self.t.items():
+            setattr(self.d, name, ten.data_ptr())
+        self.d.f = self._pops["fa"].data_ptr()
+        self.d.f_next = self._pops["fb"].data_ptr()
+        self.d.g = self._pops["ga"].data_ptr()
+        self.d.g_next = self._pops["gb"].data_ptr()
+        self.d.owned = 0
+        self.step_index = 0
+        self._graph: Optional[C.c_void_p] = None
+        torch.cuda.synchronize(self.device)  # zero fills above ran on the current stream
+
+    # -- stream -------------------------------------------------------------
+    @property
+    def stream_handle(self) -> int:
+        return self.stream.cuda_stream
+
+    def synchronize(self) -> None:
+        self.stream.synchronize()
+
+    # -- lifecycle ----------------------------------------------------------
+    def init(self) -> None:
+        abi.check(self.lib.ssmd_init(C.byref(self.p), C.byref(self.d), self.stream_handle), "ssmd_init")
+        self.step_index = 0
+
+    def step(self, n: int = 1, macros: bool = False) -> None:
+        """n eager steps; with macros=True the last one writes every moment field."""
+        for i in range(n):
+            flags = abi.STEP_MACROS if (macros and i == n - 1) else 0
+            abi.check(self.lib.ssmd_step(C.byref(self.p), C.byref(self.d), self.step_index, flags,
+                                         self.stream_handle), "ssmd_step")
+            self.step_index += 1
+
+    # the three launches of a step, for timing them separately
+    def launch_phase(self) -> None:
+        abi.check(self.lib.ssmd_launch_phase(C.byref(self.p), C.byref(self.d), self.stream_handle), "phase")
+
+    def launch_lattice(self, macros: bool = False) -> None:
+        abi.check(self.lib.ssmd_launch_lattice(C.byref(self.p), C.byref(self.d), abi.STEP_MACROS if macros else 0,
+                                               self.stream_handle), "lattice")
+
+    def launch_boundaries(self) -> None:
+        abi.check(self.lib.ssmd_launch_boundaries(C.byref(self.p), C.byref(self.d), self.step_index, 0,
+                                                  self.stream_handle), "boundaries")
+
+    def swap(self) -> None:
+        abi.check(self.lib.ssmd_swap(C.byref(self.d)), "swap")
+        self.step_index += 1
+
+    def graph_run(self, n: int) -> None:
+        """n steps replayed from captured HIP graphs (boundaries included)."""
+        if self._graph is None:
+            h = C.c_void_p()
+            abi.check(self.lib.ssmd_graph_create(C.byref(self.p), C.byref(self.d), self.stream_handle, C.byref(h)),
+                      "ssmd_graph_create")
+            self._graph = h
+        abi.check(self.lib.ssmd_graph_launch(self._graph, C.byref(self.d), self.step_index, n, self.stream_handle),
+                  "ssmd_graph_launch")
+        self.step_index += n
+
+    def close(self) -> None:
+        if self._graph is not None:
+            self.lib.ssmd_graph_destroy(self._graph)
+            self._graph = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- readback -------------------------------------------------------------
+    def field(self, name: str) -> np.ndarray:
+        """A scalar field as float64 numpy (nz, ny, nx)."""
+        self.synchronize()
+        return self.t[name].double().cpu().numpy().reshape(self.p.shape)
+
+    def fields(self, names: Iterable[str] = ("rho", "ux", "uy", "uz", "phi")) -> Dict[str, np.ndarray]:
+        return {n: self.field(n) for n in names}
+
+    def _current(self, which: str) -> torch.Tensor:
+        ptr = self.d.f if which == "f" else self.d.g
+        for key in (("fa", "fb") if which == "f" else ("ga", "gb")):
+            if self._pops[key].data_ptr() == ptr:
+                return self._pops[key]
+        raise abi.SsmdError("population buffer bookkeeping lost")
+
+    def pops_source(self, which: str = "f") -> np.ndarray:
+        """Populations the next step reads, source-indexed, float64 (Q, nz, ny, nx)."""
+        self.synchronize()
+        q = int(self.p.q) if which == "f" else 7
+        return self._current(which).double().cpu().numpy().reshape((q,) + self.p.shape)
+
+    def set_pops_source(self, which: str, values: np.ndarray) -> None:
+        """Overwrite both buffers of f or g (source layout) with `values`."""
+        self.synchronize()
+        keys = ("fa", "fb") if which == "f" else ("ga", "gb")
+        src = torch.from_numpy(np.ascontiguousarray(values, dtype=np.float64).reshape(-1))
+        for k in keys:
+            self._pops[k].copy_(src.to(self._pops[k].dtype).to(self.device))
+        torch.cuda.synchronize(self.device)
+
+    def pops_arrival(self, which: str = "f") -> np.ndarray:
+        """Same populations in the reference's arrival-indexed layout; entries
+        whose source cell lies outside the domain are NaN."""
+        return source_to_arrival(self.pops_source(which))
+
+    def phase_mass(self) -> np.ndarray:
+        """FP64 [sum phi, sum rho ux, sum rho uy, sum rho uz] over interior cells."""
+        out = (C.c_double * 4)()
+        abi.check(self.lib.ssmd_phase_mass_f64(C.byref(self.p), C.byref(self.d), out, self.stream_handle),
+                  "ssmd_phase_mass_f64")
+        return np.array(out[:], dtype=np.float64)
+
+
+def source_to_arrival(src: np.ndarray) -> np.ndarray:
+    """arrival[i][a] = source[i][a - c_i] (NaN where a - c_i is outside)."""
+    q = src.shape[0]
+    out = np.full(src.shape, np.nan)
+    nz, ny, nx = src.shape[1:]
+    for i in range(q):
+        cx, cy, cz = DIRECTIONS[i]
+        # a = s + c  ->  a in [max(c,0), n + min(c,0))
+        az = slice(max(cz, 0), nz + min(cz, 0))
+        ay = slice(max(cy, 0), ny + min(cy, 0))
+        ax = slice(max(cx, 0), nx + min(cx, 0))
+        sz = slice(max(-cz, 0), nz + min(-cz, 0))
+        sy = slice(max(-cy, 0), ny + min(-cy, 0))
+        sx = slice(max(-cx, 0), nx + min(-cx, 0))
+        out[i][az, ay, ax] = src[i][sz, sy, sx]
+    return out
+
+
+def arrival_to_source(arr: np.ndarray) -> np.ndarray:
+    """Inverse of source_to_arrival (0 where the arrival is outside)."""
+    q = arr.shape[0]
+    out = np.zeros(arr.shape)
+    nz, ny, nx = arr.shape[1:]
+    for i in range(q):
+        cx, cy, cz = DIRECTIONS[i]
+        az = slice(max(cz, 0), nz + min(cz, 0))
+        ay = slice(max(cy, 0), ny + min(cy, 0))
+        ax = slice(max(cx, 0), nx + min(cx, 0))
+        sz = slice(max(-cz, 0), nz + min(-cz, 0))
+        sy = slice(max(-cy, 0), ny + min(-cy, 0))
+        sx = slice(max(-cx, 0), nx + min(-cx, 0))
+        out[i][sz, sy, sx] = arr[i][az, ay, ax]
+    return out

@@ -1,0 +1,114 @@
+"""C ABI checks that need no GPU: exports, struct layout, host-side logic."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from ssmdlbm_amd import abi, cases
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "ssmd.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(ssmd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = abi.load()
+    names = declared_functions()
+    assert len(names) >= 19
+    for n in names:
+        assert hasattr(lib, n), n
+    # the binding covers the whole header
+    assert sorted(abi.SIGNATURES) == names
+
+
+def test_struct_layout_matches_c(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text("""
+#include <stdio.h>
+#include <stddef.h>
+#include "ssmd.h"
+int main(void) {
+  printf("%zu %zu %zu\\n", sizeof(ssmd_jet), sizeof(ssmd_params), sizeof(ssmd_fields));
+  printf("%zu %zu %zu %zu\\n", offsetof(ssmd_params, visc_water), offsetof(ssmd_params, seed),
+         offsetof(ssmd_params, water), offsetof(ssmd_params, oil));
+  printf("%zu %zu %zu\\n", offsetof(ssmd_fields, f), offsetof(ssmd_fields, step_counter), offsetof(ssmd_fields, owned));
+  return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = [list(map(int, l.split())) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                               check=True).stdout.splitlines()]
+    assert out[0] == [C.sizeof(abi.Jet), C.sizeof(abi.Params), C.sizeof(abi.Fields)]
+    assert out[1] == [abi.Params.visc_water.offset, abi.Params.seed.offset, abi.Params.water.offset,
+                      abi.Params.oil.offset]
+    assert out[2] == [abi.Fields.f.offset, abi.Fields.step_counter.offset, abi.Fields.owned.offset]
+
+
+def test_reference_default_constants():
+    # constants.cuh:70-101, globalFunctions.cuh:170-283
+    p = cases.reference_default(19)
+    assert (p.nx, p.ny, p.nz) == (128, 256, 256)
+    f32 = np.float32
+    assert p.visc_water == pytest.approx(float(f32(0.05)) * 13 / 1400, rel=1e-15)
+    assert p.visc_oil == pytest.approx(float(f32(0.05)) * 13 / 450, rel=1e-15)
+    assert p.sigma == float((f32(0.05) * f32(0.05) * f32(13)) / f32(500))
+    assert p.gamma == 0.5
+    assert p.water.r2 == 36.0 and p.oil.r2 == 36.0  # diameter 13 -> radius 6
+    assert p.water.center_a == 63.5 and p.oil.center_a == 63.5
+    assert p.water.center_b == float(f32(0.7) * f32(127.5))
+    assert p.oil.center_b == float(f32(0.5) * f32(127.5))
+    assert (p.water.phi, p.oil.phi) == (0.0, 1.0)
+    assert p.outflow_y == 1 and p.outflow_z == 1 and p.wrap_x == abi.WRAP_REFERENCE and p.wrap_y == 0
+    assert abi.load().ssmd_validate(p) == 0
+
+
+def test_jet_case_constants():
+    p = cases.JET_D3Q19_SMALL.params()
+    assert p.visc_water == p.visc_oil == pytest.approx(1 / 60)
+    assert p.gamma == pytest.approx(0.3)  # 4 (0.8 - 1/2) / 4
+    assert p.oil.r2 == 16.0 and p.oil.center_a == 31.5 and p.oil.center_b == 31.5
+    assert cases.JET_D3Q27.params().oil.r2 == 64.0
+    assert cases.JET_D3Q19_FP16.params().oil.r2 == 144.0
+    assert cases.algorithmic_bytes_per_cell(19, abi.POP_F32) == 216
+    assert cases.algorithmic_bytes_per_cell(27, abi.POP_F32) == 280
+    assert cases.algorithmic_bytes_per_cell(19, abi.POP_F16) == 140
+    assert cases.lattice_kernel_bytes_per_cell(27, abi.POP_F32) == 248
+
+
+def test_validation_rejects_bad_parameters():
+    lib = abi.load()
+    p = cases.JET_D3Q19_SMALL.params()
+    for field, val in (("q", 15), ("pop_format", 3), ("nx", 4), ("wrap_x", 3)):
+        b = p.copy()
+        setattr(b, field, val)
+        assert lib.ssmd_validate(b) == -1, field
+    b = p.copy()
+    b.wrap_z = abi.WRAP_TRUE  # periodic z with an inlet on z = 0
+    assert lib.ssmd_validate(b) == -1
+    b = p.copy()
+    b.nx, b.ny, b.nz = 2048, 2048, 1100  # beyond 32-bit cell indices
+    assert lib.ssmd_validate(b) == -2
+    assert lib.ssmd_bytes(p, 1) == 19 * 4 * p.cells
+    b = p.copy()
+    b.pop_format = abi.POP_F16
+    assert lib.ssmd_bytes(b, 1) == 19 * 2 * p.cells and lib.ssmd_bytes(b, 2) == 7 * 4 * p.cells
+
+
+def test_error_paths_without_device():
+    lib = abi.load()
+    p = cases.JET_D3Q19_SMALL.params()
+    d = abi.Fields()  # no buffers
+    assert lib.ssmd_step(p, d, 0, 0, None) == -3
+    assert lib.ssmd_init(p, d, None) == -3
+    assert lib.ssmd_graph_launch(None, d, 0, 1, None) == -1
+    assert lib.ssmd_error_string(-3) == b"fields or graph not ready"
+    with pytest.raises(abi.SsmdError):
+        abi.check(-2, "x")

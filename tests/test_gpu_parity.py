@@ -1,0 +1,222 @@
+"""GPU parity of the HIP hot path (through the C ABI) against the CPU oracle.
+
+Tolerances (FP32 kernels vs the FP64 oracle):
+  * single step: fields within 2e-6 relative L2, populations within 2e-7 absolute;
+  * many steps: rho, u, phi within 1e-5 relative L2 (the acceptance bound);
+  * truly periodic closed box: phase mass drift <= 1e-6 over 1000 steps;
+  * graph replay == eager steps, bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import cpu_threads
+from oracle.oracle import Oracle, load as load_oracle
+from ssmdlbm_amd import abi, cases
+from ssmdlbm_amd.solver import Simulation
+
+pytestmark = pytest.mark.gpu
+
+MACRO = ("rho", "ux", "uy", "uz", "phi")
+ALL = MACRO + ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz", "fsx", "fsy", "fsz")
+
+
+def jet(**kw):
+    args = dict(nx=24, ny=22, nz=30, diameter=8)
+    args.update(kw)
+    return cases.JET_D3Q19_SMALL.with_(**args).params()
+
+
+def default_case(q, nx=32, ny=48, nz=48):
+    """The reference's own two-jet BC set on a reduced box."""
+    p = cases.reference_default(q)
+    p.nx, p.ny, p.nz = nx, ny, nz
+    cx, cy, cz = (nx - 1) * 0.5, (ny - 1) * 0.5, (nz - 1) * 0.5
+    p.water.center_a = cx
+    p.water.center_b = float(np.float32(0.7) * np.float32(cz))
+    p.oil.center_a = cx
+    p.oil.center_b = float(np.float32(0.5) * np.float32(cy))
+    return p
+
+
+def run_gpu(p, steps, graph=False):
+    sim = Simulation(p)
+    sim.init()
+    if steps > 1:
+        sim.graph_run(steps - 1) if graph else sim.step(steps - 1)
+    if steps >= 1:
+        sim.step(1, macros=True)
+    sim.synchronize()
+    return sim
+
+
+def run_oracle(p, steps, fp64=True):
+    o = Oracle(p, fp64=fp64, threads=cpu_threads())
+    o.init()
+    o.step(steps)
+    return o
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def interior_arrivals(a):
+    m = np.zeros(a.shape, dtype=bool)
+    m[:, 1:-1, 1:-1, 1:-1] = True
+    return m
+
+
+# -- storage conversions -------------------------------------------------------
+@pytest.mark.parametrize("fmt", [abi.POP_F16, abi.POP_BF16])
+def test_quantize_matches_ieee_rne(gpu, fmt):
+    L = load_oracle()
+    rng = np.random.default_rng(7)
+    x = (rng.standard_normal(1 << 16) * 10.0 ** rng.integers(-12, 6, 1 << 16)).astype(np.float32)
+    x[:8] = [0.0, -0.0, 1.0, 65504.0, 65520.0, 2.0 ** -25, 3 * 2.0 ** -25, 1 + 2.0 ** -11]
+    t_in = torch.from_numpy(x).to(gpu)
+    t_out = torch.empty_like(t_in)
+    abi.check(abi.load().ssmd_quantize(fmt, t_in.data_ptr(), t_out.data_ptr(), x.size,
+                                       torch.cuda.current_stream().cuda_stream), "quantize")
+    got = t_out.cpu().numpy()
+    if fmt == abi.POP_F16:
+        want = np.array([L.oracle_f16_to_f32(L.oracle_f32_to_f16(float(v))) for v in x], dtype=np.float32)
+    else:
+        want = np.array([L.oracle_bf16_to_f32(L.oracle_f32_to_bf16(float(v))) for v in x], dtype=np.float32)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+# -- t = 0 ----------------------------------------------------------------------
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("which", ["jet", "default"])
+def test_init_state(gpu, q, which):
+    p = jet(q=q) if which == "jet" else default_case(q)
+    sim = run_gpu(p, 0)
+    o = run_oracle(p, 0)
+    for name in MACRO:
+        np.testing.assert_allclose(sim.field(name), o.field(name), rtol=0, atol=1e-7)
+    for w in ("f", "g"):
+        a = sim.pops_arrival(w)
+        b = o.pops(w)
+        m = ~np.isnan(a)
+        np.testing.assert_allclose(a[m], b[m], rtol=0, atol=1e-8)
+
+
+# -- one step --------------------------------------------------------------------
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("which", ["jet", "default"])
+def test_single_step(gpu, q, which):
+    p = jet(q=q, perturb=0.01, seed=11) if which == "jet" else default_case(q)
+    # two steps: the first fills the boundary-adjacent planes, the second
+    # exercises every kernel on a non-trivial state
+    sim = run_gpu(p, 2)
+    o = run_oracle(p, 2)
+    for name in ALL:
+        a, b = sim.field(name), o.field(name)
+        assert rel_l2(a, b) < 2e-6, (name, rel_l2(a, b))
+    for w in ("f", "g"):
+        a, b = sim.pops_arrival(w), o.pops(w)
+        m = interior_arrivals(a)
+        assert np.nanmax(np.abs(a[m] - b[m])) < 2e-7, w
+
+
+# -- many steps ------------------------------------------------------------------
+@pytest.mark.parametrize("q,which,steps", [(19, "jet", 300), (27, "jet", 300), (19, "default", 200),
+                                           (27, "default", 200)])
+def test_multi_step(gpu, q, which, steps):
+    p = jet(q=q, perturb=0.01, seed=3) if which == "jet" else default_case(q)
+    sim = run_gpu(p, steps)
+    o = run_oracle(p, steps)
+    for name in MACRO:
+        e = rel_l2(sim.field(name), o.field(name))
+        assert e < 1e-5, (name, e)
+
+
+@pytest.mark.slow
+def test_jet_d3q19_64x64x128_1000_steps(gpu):
+    """Acceptance: plain FP32 D3Q19 + D3Q7 jet, 1000 steps, rel L2 <= 1e-5 vs FP64."""
+    p = cases.JET_D3Q19_SMALL.params()
+    sim = run_gpu(p, 1000, graph=True)
+    o = run_oracle(p, 1000)
+    errs = {n: rel_l2(sim.field(n), o.field(n)) for n in MACRO}
+    print("rel L2 after 1000 steps:", errs)
+    for n, e in errs.items():
+        assert e < 1e-5, (n, e)
+
+
+# -- wraps -----------------------------------------------------------------------
+@pytest.mark.parametrize("q", [19, 27])
+def test_true_periodic_wrap(gpu, q):
+    p = jet(q=q, perturb=0.01, wrap=abi.WRAP_TRUE)
+    sim = run_gpu(p, 60)
+    o = run_oracle(p, 60)
+    for name in MACRO:
+        assert rel_l2(sim.field(name), o.field(name)) < 1e-5, name
+
+
+def droplet_box(q, n=32):
+    p = cases.closed_box(n, q=q, sigma=0.02)
+    return p
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_closed_box_phase_mass_conserved(gpu, q):
+    n = 32
+    p = droplet_box(q, n)
+    sim = Simulation(p)
+    sim.init()
+    # droplet: set g = w_k phi at every cell (the source layout of a state at rest)
+    z, y, x = np.meshgrid(*[np.arange(float(n))] * 3, indexing="ij")
+    c = (n - 1) / 2
+    r = np.sqrt((x - c - 1.3) ** 2 + (y - c) ** 2 + (z - c + 0.7) ** 2)
+    phi = 0.5 * (1 - np.tanh(2 * (r - 8) / 4))
+    w7 = np.array([0.25] + [0.125] * 6)
+    sim.set_pops_source("g", w7[:, None, None, None] * phi[None])
+    sim.step(1)
+    m0 = sim.phase_mass()[0]
+    sim.graph_run(1000)
+    sim.step(1, macros=True)
+    m1 = sim.phase_mass()
+    assert abs(m1[0] / m0 - 1) <= 1e-6, (m0, m1[0])
+    np.testing.assert_allclose(m1[1:], 0, atol=1e-6 * n ** 3)
+
+
+# -- graphs, reductions ------------------------------------------------------------
+@pytest.mark.parametrize("q", [19, 27])
+def test_graph_replay_matches_eager_bitwise(gpu, q):
+    p = jet(q=q, perturb=0.01, seed=9)
+    a = run_gpu(p, 41, graph=False)
+    b = run_gpu(p, 41, graph=True)
+    for name in MACRO:
+        np.testing.assert_array_equal(a.field(name), b.field(name))
+    np.testing.assert_array_equal(a.pops_source("f"), b.pops_source("f"))
+
+
+def test_phase_mass_reduction(gpu):
+    p = jet(perturb=0.01)
+    sim = run_gpu(p, 5)
+    got = sim.phase_mass()
+    f = sim.fields()
+    inner = (slice(1, -1),) * 3
+    want = [f["phi"][inner].sum()] + [(f["rho"] * f[k])[inner].sum() for k in ("ux", "uy", "uz")]
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-9)
+
+
+# -- precision forms ----------------------------------------------------------------
+@pytest.mark.parametrize("fmt,tol", [(abi.POP_F16, 2e-3), (abi.POP_BF16, 2e-2)])
+def test_reduced_precision_storage(gpu, fmt, tol):
+    """16-bit population storage: error vs the FP64 oracle stated per field."""
+    base = jet(perturb=0.01, seed=4)
+    p = base.copy()
+    p.pop_format = fmt
+    steps = 300
+    sim = run_gpu(p, steps)
+    o64 = run_oracle(base, steps)
+    o16 = run_oracle(p, steps, fp64=False)  # float arithmetic, same storage rounding
+    errs = {n: rel_l2(sim.field(n), o64.field(n)) for n in MACRO}
+    oerr = {n: rel_l2(o16.field(n), o64.field(n)) for n in MACRO}
+    print("storage", fmt, "GPU vs FP64:", errs, "oracle-16 vs FP64:", oerr)
+    for n in MACRO:
+        assert errs[n] < tol, (n, errs[n])
+        assert errs[n] < 3 * oerr[n] + 1e-6, (n, errs[n], oerr[n])
