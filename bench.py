@@ -113,6 +113,26 @@ def cpu_baseline(case, budget_s: float = 15.0):
     }
 
 
+def profile_mode(case, steps: int) -> int:
+    """Eager steps (one dispatch per kernel, as counter collection wants them),
+    then torch copies of 2^28 floats = 1 GiB read + 1 GiB written."""
+    import torch
+
+    from ssmdlbm_amd.solver import Simulation
+
+    sim = Simulation(case.params(), "cuda:0")
+    sim.init()
+    sim.step(steps)
+    sim.synchronize()
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"profile mode: {steps} steps of {case.name}, 3 copies of 1 GiB")
+    return 0
+
+
 def load_traffic(workload: str):
     path = ROOT / "profiles" / f"k_lattice_{workload}_traffic.json"
     if path.exists():
@@ -133,6 +153,9 @@ def main() -> int:
     ap.add_argument("--form-steps", type=int, default=300)
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16-vs-FP32 storage measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="profiling mode for rocprofv3 --pmc: N eager steps of the workload, then a plain "
+                         "1 GiB device copy (the byte-count reference), no timing output")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -149,6 +172,8 @@ def main() -> int:
         return 1
     abi.load()
     case = cases.WORKLOADS[args.workload]
+    if args.profile_steps:
+        return profile_mode(case, args.profile_steps)
     log(f"workload {case.name}: warmup {args.warmup}, timed {args.steps}")
     main_m = measure(case, args.steps, args.warmup, args.kernel_steps)
     log(f"main: {main_m}")

@@ -84,6 +84,9 @@ typedef struct ssmd_params {
     int32_t outflow_y;   /* applyOutflowY on y = ny-1 */
     int32_t outflow_z;   /* applyOutflowZ on z = nz-1 */
     int32_t wrap_x, wrap_y, wrap_z; /* SSMD_WRAP_* */
+    int32_t tile;        /* lattice tile shape: index into the built shapes
+                            (0: 64x4, 1: 128x2, 2: 64x2, 3: 128x4); see ssmd_tune */
+    int32_t lz;          /* z planes per lattice block; 0 = automatic */
     double visc_water;   /* relaxation::visc_water() */
     double visc_oil;     /* relaxation::visc_oil() */
     double sigma;        /* physics::sigma */
@@ -182,6 +185,49 @@ int ssmd_phase_mass_f64(const ssmd_params *p, const ssmd_fields *d, double *out_
 /* Bit-exact storage conversions used by the kernels, evaluated on the device
  * (n elements, device pointers): float32 -> format -> float32. */
 int ssmd_quantize(int32_t pop_format, const float *in, float *out, uint64_t n, void *stream);
+
+/* Autotuner: times the fused lattice kernel for every built tile shape and a
+ * set of z-chunk lengths on the current state of *d (reps launches each) and
+ * returns the fastest in *best_tile, *best_lz (ready for p->tile, p->lz).
+ * The state the next step reads (f, g) is not changed; f_next, g_next, phi
+ * and the face-plane moments are rewritten as a step's launches would. */
+int ssmd_tune(const ssmd_params *p, const ssmd_fields *d, int32_t reps, void *stream, int32_t *best_tile,
+              int32_t *best_lz);
+int ssmd_tile_count(void);
+int ssmd_tile_shape(int32_t tile, int32_t *tx, int32_t *ty);
+
+/* Ensembles (batching form): n independent field sets of uneven size and
+ * parameters -- the same lattice, storage format and boundary set -- advanced
+ * by ONE launch per kernel kind per step: a persistent lattice kernel striding
+ * over the tiles of every member, and batched phase and face kernels.  Each
+ * member's results are bit-identical to running it alone.  The ensemble
+ * keeps its own copy of the field descriptors (swapped each step); read the
+ * current ones with ssmd_ensemble_fields.  Graph replays advance a device
+ * step counter owned by the ensemble. */
+typedef struct ssmd_ensemble ssmd_ensemble;
+int ssmd_ensemble_create(const ssmd_params *params, const ssmd_fields *fields, int32_t n, ssmd_ensemble **out);
+int ssmd_ensemble_step(ssmd_ensemble *e, uint64_t step, uint32_t flags, void *stream);
+int ssmd_ensemble_graph_launch(ssmd_ensemble *e, uint64_t step0, uint64_t nsteps, void *stream);
+/* Autotune the ensemble's lattice tile shape (kernel-only timing, as ssmd_tune). */
+int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *best_tile);
+int ssmd_ensemble_set_tile(ssmd_ensemble *e, int32_t tile);
+int ssmd_ensemble_fields(const ssmd_ensemble *e, int32_t member, ssmd_fields *out);
+int ssmd_ensemble_destroy(ssmd_ensemble *e);
+
+/* Output in the reference's formats (host code; synchronizes `stream`):
+ *   ssmd_write_bin   <dir>/<name><step:06>.bin, raw float32 of a device field
+ *                    (host::PostProcess::bin, postProcess/PostProcess.cuh:45-98)
+ *   ssmd_write_vti   <dir>/step_<step:06>.vti from those .bin files: VTK
+ *                    ImageData, appended raw, uint32 byte count per array;
+ *                    missing or wrong-size .bin files are skipped
+ *                    (host::PostProcess::vti, PostProcess.cuh:101-262)
+ *   ssmd_write_info  <dir>/<id>_info.txt metadata
+ *                    (host::generateSimulationInfoFile, functions/hostFunctions.cuh:47-87) */
+int ssmd_write_bin(const ssmd_params *p, const float *device_field, const char *dir, const char *name, uint64_t step,
+                   void *stream);
+int ssmd_write_vti(const ssmd_params *p, const char *dir, uint64_t step, const char *const *names, int32_t count);
+int ssmd_write_info(const ssmd_params *p, const char *dir, const char *id, const char *velocity_set, uint64_t nsteps,
+                    uint64_t save_every);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,9 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    lib.oracle_params_size.restype = C.c_size_t
+    if lib.oracle_params_size() != C.sizeof(abi.Params):
+        raise RuntimeError("oracle built against a different ssmd_params layout; rebuild with oracle/build.py")
     _lib = lib
     return lib
 

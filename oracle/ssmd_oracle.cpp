@@ -686,6 +686,8 @@ struct oracle_handle {
     std::unique_ptr<Base> impl;
 };
 
+size_t oracle_params_size(void) { return sizeof(ssmd_params); }
+
 oracle_handle *oracle_create(const ssmd_params *p, int fp64) {
     if (!p || (p->q != 19 && p->q != 27) || p->nx < 3 || p->ny < 3 || p->nz < 3) return nullptr;
     auto *h = new oracle_handle;

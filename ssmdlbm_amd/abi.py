@@ -41,6 +41,8 @@ class Params(C.Structure):
         ("wrap_x", C.c_int32),
         ("wrap_y", C.c_int32),
         ("wrap_z", C.c_int32),
+        ("tile", C.c_int32),
+        ("lz", C.c_int32),
         ("visc_water", C.c_double),
         ("visc_oil", C.c_double),
         ("sigma", C.c_double),
@@ -103,6 +105,20 @@ SIGNATURES = {
     "ssmd_graph_destroy": (C.c_int, [C.c_void_p]),
     "ssmd_phase_mass_f64": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.POINTER(C.c_double), C.c_void_p]),
     "ssmd_quantize": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "ssmd_tune": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_int32, C.c_void_p, C.POINTER(C.c_int32),
+                            C.POINTER(C.c_int32)]),
+    "ssmd_tile_count": (C.c_int, []),
+    "ssmd_tile_shape": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ssmd_ensemble_create": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_int32, C.POINTER(C.c_void_p)]),
+    "ssmd_ensemble_step": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),
+    "ssmd_ensemble_graph_launch": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "ssmd_ensemble_tune": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
+    "ssmd_ensemble_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ssmd_ensemble_fields": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Fields)]),
+    "ssmd_ensemble_destroy": (C.c_int, [C.c_void_p]),
+    "ssmd_write_bin": (C.c_int, [C.POINTER(Params), C.c_void_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_void_p]),
+    "ssmd_write_vti": (C.c_int, [C.POINTER(Params), C.c_char_p, C.c_uint64, C.POINTER(C.c_char_p), C.c_int32]),
+    "ssmd_write_info": (C.c_int, [C.POINTER(Params), C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64]),
 }
 
 

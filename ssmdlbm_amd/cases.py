@@ -140,3 +140,25 @@ def lattice_kernel_bytes_per_cell(q: int, pop_format: int) -> int:
     """Algorithmic bytes of the fused lattice kernel alone: f read + f write,
     g write, phi read."""
     return 2 * q * abi.POP_BYTES[pop_format] + 7 * 4 + 4
+
+
+def ensemble(n_jets: int = 32, seed: int = 4, q: int = 19, pop_format: int = abi.POP_F32) -> list[JetCase]:
+    """Seeded ensemble of single jets with uneven boxes and parameters.
+
+    Boxes: nx = ny uniform in {48, 56, ..., 128}, nz uniform in {96, 104, ..., 384};
+    inlet u_z ~ U[0.02, 0.08], sigma ~ U[0.01, 0.1], diameter ~ U{6..16};
+    the inlet-noise seed of jet j is seed * 1000 + j.
+    """
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for j in range(n_jets):
+        nxy = int(rng.integers(6, 17)) * 8
+        nz = int(rng.integers(12, 49)) * 8
+        u = float(rng.uniform(0.02, 0.08))
+        sigma = float(rng.uniform(0.01, 0.1))
+        d = int(rng.integers(6, 17))
+        out.append(JetCase(f"ens{seed}_{j:02d}", nxy, nxy, nz, q=q, sigma=sigma, diameter=d, u=u,
+                           seed=seed * 1000 + j, pop_format=pop_format))
+    return out

@@ -1,0 +1,131 @@
+// io.cpp -- host-side output of the solver fields, same files as the
+// reference's post-processing:
+//   ssmd_write_bin   host::PostProcess::bin  (postProcess/PostProcess.cuh:45-98):
+//                    one raw float32 file <dir>/<name><step:06>.bin per field
+//   ssmd_write_vti   host::PostProcess::vti  (PostProcess.cuh:101-262): VTK
+//                    ImageData step_<step:06>.vti, appended raw encoding, a
+//                    uint32 byte count before each array, re-reading the .bin files
+//   ssmd_write_info  host::generateSimulationInfoFile (functions/hostFunctions.cuh:47-87)
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ssmd.h"
+
+namespace {
+
+std::string step6(uint64_t step) {
+    std::ostringstream ss;
+    ss << std::setw(6) << std::setfill('0') << step;
+    return ss.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssmd_write_bin(const ssmd_params *p, const float *device_field, const char *dir, const char *name, uint64_t step,
+                   void *stream) {
+    if (!p || !device_field || !dir || !name) return SSMD_EINVAL;
+    if (ssmd_validate(p) != 0) return SSMD_EINVAL;
+    const size_t n = size_t(p->nx) * p->ny * p->nz;
+    std::vector<float> h(n);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(h.data(), device_field, n * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return int(e);
+    const std::filesystem::path out = std::filesystem::path(dir) / (std::string(name) + step6(step) + ".bin");
+    std::ofstream f(out, std::ios::binary | std::ios::trunc);
+    if (!f) return SSMD_EINVAL;
+    f.write(reinterpret_cast<const char *>(h.data()), std::streamsize(n * sizeof(float)));
+    return f ? SSMD_OK : SSMD_EINVAL;
+}
+
+int ssmd_write_vti(const ssmd_params *p, const char *dir, uint64_t step, const char *const *names, int32_t count) {
+    if (!p || !dir || !names || count <= 0) return SSMD_EINVAL;
+    const uint64_t node_bytes = uint64_t(p->nx) * p->ny * p->nz * sizeof(float);
+    struct Arr {
+        std::string name;
+        std::filesystem::path path;
+        uint64_t offset;
+    };
+    std::vector<Arr> arrays;
+    uint64_t off = 0;
+    for (int i = 0; i < count; ++i) {
+        const std::filesystem::path bin = std::filesystem::path(dir) / (std::string(names[i]) + step6(step) + ".bin");
+        std::error_code ec;
+        if (!std::filesystem::exists(bin, ec) || ec) continue;  // missing .bin: skipped, as the reference
+        const uint64_t fs = std::filesystem::file_size(bin, ec);
+        if (ec || fs != node_bytes) continue;  // wrong size: skipped
+        arrays.push_back({names[i], bin, off});
+        off += sizeof(uint32_t) + node_bytes;
+    }
+    if (arrays.empty()) return SSMD_EINVAL;
+    const std::filesystem::path out = std::filesystem::path(dir) / ("step_" + step6(step) + ".vti");
+    std::ofstream v(out, std::ios::binary | std::ios::trunc);
+    if (!v) return SSMD_EINVAL;
+    const double zero = 0.0, one = 1.0;
+    v << "<?xml version=\"1.0\"?>\n";
+    v << "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+    v << "  <ImageData WholeExtent=\"0 " << (p->nx - 1) << " 0 " << (p->ny - 1) << " 0 " << (p->nz - 1)
+      << "\" Origin=\"" << zero << " " << zero << " " << zero << "\" Spacing=\"" << one << " " << one << " " << one
+      << "\">\n";
+    v << "    <Piece Extent=\"0 " << (p->nx - 1) << " 0 " << (p->ny - 1) << " 0 " << (p->nz - 1) << "\">\n";
+    v << "      <PointData Scalars=\"" << arrays.front().name << "\">\n";
+    for (const Arr &a : arrays)
+        v << "        <DataArray type=\"Float32\" Name=\"" << a.name
+          << "\" NumberOfComponents=\"1\" format=\"appended\" offset=\"" << a.offset << "\"/>\n";
+    v << "      </PointData>\n";
+    v << "      <CellData/>\n";
+    v << "    </Piece>\n";
+    v << "  </ImageData>\n";
+    v << "  <AppendedData encoding=\"raw\">\n";
+    v << '_';
+    std::vector<char> buf(1 << 20);
+    for (const Arr &a : arrays) {
+        const uint32_t nb = uint32_t(node_bytes);
+        v.write(reinterpret_cast<const char *>(&nb), sizeof(nb));
+        std::ifstream in(a.path, std::ios::binary);
+        while (in) {
+            in.read(buf.data(), std::streamsize(buf.size()));
+            const std::streamsize got = in.gcount();
+            if (got > 0) v.write(buf.data(), got);
+        }
+    }
+    v << "\n  </AppendedData>\n";
+    v << "</VTKFile>\n";
+    return v ? SSMD_OK : SSMD_EINVAL;
+}
+
+int ssmd_write_info(const ssmd_params *p, const char *dir, const char *id, const char *velocity_set, uint64_t nsteps,
+                    uint64_t save_every) {
+    if (!p || !dir || !id || !velocity_set) return SSMD_EINVAL;
+    const std::filesystem::path path = std::filesystem::path(dir) / (std::string(id) + "_info.txt");
+    std::ofstream f(path, std::ios::out | std::ios::trunc);
+    if (!f) return SSMD_EINVAL;
+    auto diam = [](const ssmd_jet &j) { return 2 * int(std::sqrt(j.r2) + 1e-9); };
+    f << "---------------------------- SIMULATION METADATA ----------------------------\n"
+      << "ID:                 " << id << '\n'
+      << "Velocity set:       " << velocity_set << '\n'
+      << "Water velocity:     " << (p->water.enabled ? float(p->water.u) : 0.0f) << '\n'
+      << "Oil velocity:       " << (p->oil.enabled ? float(p->oil.u) : 0.0f) << '\n'
+      << "Water viscosity:    " << p->visc_water << '\n'
+      << "Oil viscosity:      " << p->visc_oil << '\n'
+      << "Surface tension:    " << float(p->sigma) << "\n\n"
+      << "Domain size:        NX=" << p->nx << ", NY=" << p->ny << ", NZ=" << p->nz << '\n'
+      << "Water diameter:     D=" << (p->water.enabled ? diam(p->water) : 0) << '\n'
+      << "Oil diameter:       D=" << (p->oil.enabled ? diam(p->oil) : 0) << '\n'
+      << "Timesteps:          " << nsteps << '\n'
+      << "Output interval:    " << save_every << "\n\n"
+      << "-----------------------------------------------------------------------------\n";
+    return f ? SSMD_OK : SSMD_EINVAL;
+}
+
+}  // extern "C"

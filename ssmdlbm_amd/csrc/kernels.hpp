@@ -52,8 +52,8 @@ __device__ __forceinline__ bool is_interior(const KParams &k, int x, int y, int 
 
 // --------------------------------------------------------------------------
 // K1: phi = sum of the seven arriving phase populations (phaseField.cuh:31-52).
-__global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi) {
-    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void phase_cell(const KParams &k, const float *__restrict__ g, float *__restrict__ phi,
+                                           uint32_t c) {
     if (c >= k.N) return;
     const int x = int(c % uint32_t(k.nx));
     const uint32_t r = c / uint32_t(k.nx);
@@ -71,26 +71,41 @@ __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restric
     phi[c] = s;
 }
 
+#ifdef SSMD_STEP_KERNELS
+__global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi) {
+    phase_cell(k, g, phi, blockIdx.x * 256u + threadIdx.x);
+}
+#endif  // SSMD_STEP_KERNELS
+
+
 // --------------------------------------------------------------------------
-// Fused lattice kernel.  Block = TX x TY threads owning an x-y tile, marching
-// over LZ planes of z.  LDS rings (5 phi planes with a 2-cell halo, masked and
-// raw; 3 planes of masked normals with a 1-cell halo) give every thread the
-// radius-2 neighbourhood the force and normal stencils need.
+// Fused lattice kernel.  A block of TX x TY threads owns an x-y tile and
+// marches over a chunk of z planes.  LDS rings (5 phi planes with a 2-cell
+// halo, masked and raw; 3 planes of masked normals with a 1-cell halo) give
+// every thread the radius-2 neighbourhood the force and normal stencils need.
+template <int TX, int TY>
+struct LatticeSmem {
+    static constexpr int HX = TX + 4, HY = TY + 4, HP = HX * HY;
+    static constexpr int NXl = TX + 2, NYl = TY + 2, NP = NXl * NYl;
+    float m[5][HP];   // phi-hat: phi on interior cells, 0 on ghosts / outside
+    float r[5][HP];   // phi as stored (0 outside the domain)
+    float4 n[3][NP];  // masked normal (x,y,z) and |grad phi-hat|
+};
+
+// One tile: cells x0..x0+TX-1, y0..y0+TY-1, planes zb..ze-1 (threadIdx = (tx, ty)).
 template <int Q, class P, int TX, int TY, bool MACROS>
-__global__ void __launch_bounds__(TX *TY)
-    k_lattice(KParams k, const typename P::T *__restrict__ fin, typename P::T *__restrict__ fout,
-              float *__restrict__ gout, const float *__restrict__ phi, Macro mac, int lz) {
-    constexpr int HX = TX + 4, HY = TY + 4, HP = HX * HY;
-    constexpr int NXl = TX + 2, NYl = TY + 2, NP = NXl * NYl;
+__device__ __forceinline__ void lattice_tile(const KParams &k, const typename P::T *__restrict__ fin,
+                                             typename P::T *__restrict__ fout, float *__restrict__ gout,
+                                             const float *__restrict__ phi, const Macro &mac, int x0, int y0,
+                                             int zb, int ze, LatticeSmem<TX, TY> &sm) {
+    using S = LatticeSmem<TX, TY>;
+    constexpr int HX = S::HX, HP = S::HP, NXl = S::NXl, NP = S::NP;
     constexpr int NT = TX * TY;
-    __shared__ float s_m[5][HP];   // phi-hat: phi on interior cells, 0 on ghosts / outside
-    __shared__ float s_r[5][HP];   // phi as stored (0 outside the domain)
-    __shared__ float4 s_n[3][NP];  // masked normal (x,y,z) and |grad phi-hat|
+    auto &s_m = sm.m;
+    auto &s_r = sm.r;
+    auto &s_n = sm.n;
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    const int zb = 1 + int(blockIdx.z) * lz;
-    const int ze = min(zb + lz, k.nz - 1);
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x >= 1 && x <= k.nx - 2 && y >= 1 && y <= k.ny - 2;
     const size_t N = k.N;
@@ -232,6 +247,77 @@ __global__ void __launch_bounds__(TX *TY)
     }
 }
 
+template <int Q, class P, int TX, int TY, bool MACROS>
+__global__ void __launch_bounds__(TX *TY)
+    k_lattice(KParams k, const typename P::T *__restrict__ fin, typename P::T *__restrict__ fout,
+              float *__restrict__ gout, const float *__restrict__ phi, Macro mac, int lz) {
+    __shared__ LatticeSmem<TX, TY> sm;
+    const int zb = 1 + int(blockIdx.z) * lz;
+    lattice_tile<Q, P, TX, TY, MACROS>(k, fin, fout, gout, phi, mac, int(blockIdx.x) * TX, int(blockIdx.y) * TY, zb,
+                                       min(zb + lz, k.nz - 1), sm);
+}
+
+// --------------------------------------------------------------------------
+// Ensembles: one launch covers every jet.  Work units (tiles, cell blocks,
+// face blocks) of jet j occupy the global id range [begin_j, begin_{j+1});
+// a block looks its jet up in the table (uniform, <= 64 entries).
+struct JetSlot {
+    KParams k;
+    const void *f;  // populations read by the step
+    void *f_next;
+    const float *g;
+    float *g_next;
+    float *phi;
+    Macro mac;
+    int tiles_x, tiles_y, lz;  // lattice tiling of this jet
+    uint32_t tile_begin;       // first lattice tile id
+    uint32_t cell_block_begin; // first 256-cell block id (phase kernel)
+    uint32_t face_begin[3];    // first face block id for faces normal to x, y, z (64x4 blocks)
+};
+
+__device__ __forceinline__ int find_slot(const JetSlot *jets, int njets, uint32_t id, int which) {
+    int j = 0;
+    while (j + 1 < njets) {
+        const JetSlot &n = jets[j + 1];
+        const uint32_t b = which == 0 ? n.tile_begin : (which == 1 ? n.cell_block_begin : n.face_begin[which - 2]);
+        if (b > id) break;
+        ++j;
+    }
+    return j;
+}
+
+#ifdef SSMD_STEP_KERNELS
+// K1 for every jet: one 256-thread block per 256 cells of a jet.
+__global__ void __launch_bounds__(256) k_phase_batch(const JetSlot *__restrict__ jets, int njets) {
+    const int j = find_slot(jets, njets, blockIdx.x, 1);
+    const JetSlot &js = jets[j];
+    phase_cell(js.k, js.g, js.phi, (blockIdx.x - js.cell_block_begin) * 256u + threadIdx.x);
+}
+#endif  // SSMD_STEP_KERNELS
+
+
+// Persistent lattice kernel: gridDim.x blocks stride over all tiles of all jets.
+template <int Q, class P, int TX, int TY, bool MACROS>
+__global__ void __launch_bounds__(TX *TY) k_lattice_batch(const JetSlot *__restrict__ jets, int njets,
+                                                          uint32_t total_tiles) {
+    __shared__ LatticeSmem<TX, TY> sm;
+    using PT = typename P::T;
+    for (uint32_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        const int j = find_slot(jets, njets, t, 0);
+        const JetSlot &js = jets[j];
+        uint32_t lt = t - js.tile_begin;
+        const int bx = int(lt % uint32_t(js.tiles_x));
+        lt /= uint32_t(js.tiles_x);
+        const int by = int(lt % uint32_t(js.tiles_y));
+        const int bz = int(lt / uint32_t(js.tiles_y));
+        const int zb = 1 + bz * js.lz;
+        __syncthreads();  // the previous tile's LDS readers are done
+        lattice_tile<Q, P, TX, TY, MACROS>(js.k, static_cast<const PT *>(js.f), static_cast<PT *>(js.f_next),
+                                           js.g_next, js.phi, js.mac, bx * TX, by * TY, zb,
+                                           min(zb + js.lz, js.k.nz - 1), sm);
+    }
+}
+
 // --------------------------------------------------------------------------
 // Boundary kernels.  Each writes the SOURCE slots of the populations the
 // reference writes at their arrival cells, so arrival a along c_i maps to
@@ -292,11 +378,10 @@ __device__ __forceinline__ void bump(uint64_t *counter, int doit) {
 
 // applyWaterInflow (axis 1, face y=0) / applyOilInflow (axis 2, face z=0).
 template <int Q, class P, int AXIS>
-__global__ void k_inflow(KParams k, typename P::T *__restrict__ fout, float *__restrict__ gout, float *__restrict__ phi,
-                         Macro mac, const uint64_t *counter, uint64_t step) {
+__device__ __forceinline__ void inflow_face(const KParams &k, typename P::T *__restrict__ fout, float *__restrict__ gout,
+                                            float *__restrict__ phi, const Macro &mac, const uint64_t *counter,
+                                            uint64_t step, int a, int b) {
     const JetK &j = AXIS == 1 ? k.water : k.oil;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y * blockDim.y + threadIdx.y;
     const int nb = AXIS == 1 ? k.nz : k.ny;
     if (a < k.nx && b < nb) {
         const float da = float(a) - j.ca, db = float(b) - j.cb;
@@ -343,10 +428,9 @@ __global__ void k_inflow(KParams k, typename P::T *__restrict__ fout, float *__r
 
 // applyOutflowY (axis 1) / applyOutflowZ (axis 2).
 template <int Q, class P, int AXIS>
-__global__ void k_outflow(KParams k, typename P::T *__restrict__ fout, float *__restrict__ gout, float *__restrict__ phi,
-                          Macro mac, uint64_t *counter, int bump_last) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+__device__ __forceinline__ void outflow_face(const KParams &k, typename P::T *__restrict__ fout,
+                                             float *__restrict__ gout, float *__restrict__ phi, const Macro &mac,
+                                             int a, int b) {
     const int nb = AXIS == 1 ? k.nz : k.ny;
     if (a >= 1 && a <= k.nx - 2 && b >= 1 && b <= nb - 2) {
         const int x = a, y = AXIS == 1 ? k.ny - 1 : b, z = AXIS == 1 ? b : k.nz - 1;
@@ -377,18 +461,16 @@ __global__ void k_outflow(KParams k, typename P::T *__restrict__ fout, float *__
         const float un = AXIS == 1 ? uy : uz;
         gout[size_t(kk) * k.N + bnd] = weight7(kk) * ph * (1.0f - 4.0f * un);
     }
-    bump(counter, bump_last);
 }
 
 // periodicX (BoundaryConditions.cuh:295-349) and its mirrors on y and z.
 // Face coordinates (a, b) run over the two other axes in increasing order.
 template <int Q, class P, int AXIS, int MODE>
-__global__ void k_periodic(KParams k, typename P::T *__restrict__ fout, float *__restrict__ gout,
-                           float *__restrict__ phi, Macro mac, uint64_t *counter, int bump_last) {
+__device__ __forceinline__ void periodic_face(const KParams &k, typename P::T *__restrict__ fout,
+                                              float *__restrict__ gout, float *__restrict__ phi, const Macro &mac,
+                                              int a, int b) {
     constexpr int A1 = AXIS == 0 ? 1 : 0, A2 = AXIS == 2 ? 1 : 2;
     const int n[3] = {k.nx, k.ny, k.nz};
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y * blockDim.y + threadIdx.y;
     if (a < n[A1] && b < n[A2]) {
         const int nn = n[AXIS];
         auto at = [&](int s, int aa, int bb) {
@@ -440,13 +522,86 @@ __global__ void k_periodic(KParams k, typename P::T *__restrict__ fout, float *_
         mac.uz[gL] = mac.uz[bR];
         mac.uz[gR] = mac.uz[bL];
     }
-    bump(counter, bump_last);
 }
 
+// Face operations, in the order a step applies them (main.cu:165-169, then
+// the new y and z periodic copies).
+enum FaceKind {
+    kInflowY = 0,   // applyWaterInflow
+    kInflowZ,       // applyOilInflow
+    kOutflowY,      // applyOutflowY
+    kOutflowZ,      // applyOutflowZ
+    kPeriodicXRef,  // periodicX
+    kPeriodicXTrue,
+    kPeriodicYRef,
+    kPeriodicYTrue,
+    kPeriodicZRef,
+    kPeriodicZTrue,
+};
+
+constexpr int face_axis(int kind) {
+    return kind == kInflowY || kind == kOutflowY || kind == kPeriodicYRef || kind == kPeriodicYTrue
+               ? 1
+               : (kind == kPeriodicXRef || kind == kPeriodicXTrue ? 0 : 2);
+}
+
+// Extent (na, nb) of a face normal to `axis`: the two other axes in order.
+__device__ __host__ __forceinline__ void face_extent(int axis, int nx, int ny, int nz, int &na, int &nb) {
+    na = axis == 0 ? ny : nx;
+    nb = axis == 2 ? ny : nz;
+}
+
+template <int Q, class P, int KIND>
+__device__ __forceinline__ void face_op(const KParams &k, typename P::T *fout, float *gout, float *phi,
+                                        const Macro &mac, const uint64_t *counter, uint64_t step, int a, int b) {
+    if constexpr (KIND == kInflowY) inflow_face<Q, P, 1>(k, fout, gout, phi, mac, counter, step, a, b);
+    if constexpr (KIND == kInflowZ) inflow_face<Q, P, 2>(k, fout, gout, phi, mac, counter, step, a, b);
+    if constexpr (KIND == kOutflowY) outflow_face<Q, P, 1>(k, fout, gout, phi, mac, a, b);
+    if constexpr (KIND == kOutflowZ) outflow_face<Q, P, 2>(k, fout, gout, phi, mac, a, b);
+    if constexpr (KIND == kPeriodicXRef) periodic_face<Q, P, 0, 1>(k, fout, gout, phi, mac, a, b);
+    if constexpr (KIND == kPeriodicXTrue) periodic_face<Q, P, 0, 2>(k, fout, gout, phi, mac, a, b);
+    if constexpr (KIND == kPeriodicYRef) periodic_face<Q, P, 1, 1>(k, fout, gout, phi, mac, a, b);
+    if constexpr (KIND == kPeriodicYTrue) periodic_face<Q, P, 1, 2>(k, fout, gout, phi, mac, a, b);
+    if constexpr (KIND == kPeriodicZRef) periodic_face<Q, P, 2, 1>(k, fout, gout, phi, mac, a, b);
+    if constexpr (KIND == kPeriodicZTrue) periodic_face<Q, P, 2, 2>(k, fout, gout, phi, mac, a, b);
+}
+
+// One field set: 2D grid of 64x4 blocks over the face.  The inlet kinds read
+// the device step counter, so only the other kinds may advance it.
+template <int Q, class P, int KIND>
+__global__ void __launch_bounds__(256) k_face(KParams k, typename P::T *fout, float *gout, float *phi, Macro mac,
+                                              uint64_t *counter, uint64_t step, int bump_last) {
+    const int a = blockIdx.x * 64 + threadIdx.x;
+    const int b = blockIdx.y * 4 + threadIdx.y;
+    face_op<Q, P, KIND>(k, fout, gout, phi, mac, counter, step, a, b);
+    if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, bump_last);
+}
+
+// Ensemble: 1D grid of 64x4 blocks over every jet's face blocks.
+template <int Q, class P, int KIND>
+__global__ void __launch_bounds__(256) k_face_batch(const JetSlot *__restrict__ jets, int njets, uint64_t *counter,
+                                                    uint64_t step, int bump_last) {
+    constexpr int axis = face_axis(KIND);
+    const int j = find_slot(jets, njets, blockIdx.x, 2 + axis);
+    const JetSlot &js = jets[j];
+    int na, nb;
+    face_extent(axis, js.k.nx, js.k.ny, js.k.nz, na, nb);
+    const int bxs = (na + 63) / 64;
+    const uint32_t lb = blockIdx.x - js.face_begin[axis];
+    const int a = int(lb % uint32_t(bxs)) * 64 + threadIdx.x;
+    const int b = int(lb / uint32_t(bxs)) * 4 + threadIdx.y;
+    using PT = typename P::T;
+    face_op<Q, P, KIND>(js.k, static_cast<PT *>(js.f_next), js.g_next, js.phi, js.mac, counter, step, a, b);
+    if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, bump_last);
+}
+
+#ifdef SSMD_STEP_KERNELS
 __global__ void k_set_counter(uint64_t *counter, uint64_t v) { *counter = v; }
 // Advances the device step index when no outflow/periodic kernel follows
 // the inlets (the inlet kernels read the counter, so they never write it).
 __global__ void k_bump(uint64_t *counter) { *counter += 1; }
+#endif  // SSMD_STEP_KERNELS
+
 
 // --------------------------------------------------------------------------
 // t = 0 state (initialConditions.cu:33-140) in the source-indexed layout:
@@ -527,6 +682,7 @@ __global__ void __launch_bounds__(256) k_init(KParams k, typename P::T *f0, type
 // FP64 reductions over interior cells: sum phi, rho*ux, rho*uy, rho*uz.
 // Pass 1 writes one partial per block; pass 2 (one block) sums them in a
 // fixed order, so the result is deterministic.
+#ifdef SSMD_STEP_KERNELS
 __global__ void __launch_bounds__(256) k_reduce1(KParams k, const float *phi, Macro mac, double *part) {
     __shared__ double sh[4][256];
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -569,6 +725,7 @@ __global__ void __launch_bounds__(256) k_reduce2(const double *part, int nparts,
     }
     if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
 }
+#endif  // SSMD_STEP_KERNELS
 
 template <class P>
 __global__ void k_quantize(const float *in, float *out, uint64_t n) {

@@ -1,0 +1,4 @@
+// Lattice launchers for D3Q27 with bf16 population storage.
+#include "lattice_inst.hpp"
+
+SSMD_INSTANTIATE_LATTICE(27, ssmd::PopBF16)

@@ -1,0 +1,47 @@
+// launch.hpp -- host-side launchers of the fused lattice kernel, compiled
+// once per (lattice, storage format) in lattice_q*_*.hip so the heavy
+// instantiations build in parallel.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace ssmd {
+
+// Tile shapes the lattice kernels are built for (the autotuner's candidates).
+struct TileShape {
+    int tx, ty;
+};
+constexpr TileShape kTiles[] = {{64, 4}, {128, 2}, {64, 2}, {128, 4}};
+constexpr int kNumTiles = int(sizeof(kTiles) / sizeof(kTiles[0]));
+
+struct LatticeArgs {
+    KParams k;
+    const void *fin;
+    void *fout;
+    float *gout;
+    const float *phi;
+    Macro mac;
+    int lz;
+};
+
+// z planes per block: enough chunks that the grid holds >= ~2048 blocks on
+// large boxes without letting the 4-plane halo prologue dominate a chunk.
+inline int auto_lz(int nx, int ny, int nz, int tile) {
+    const TileShape t = kTiles[tile];
+    const int interior = nz - 2;
+    const long cols = long((nx + t.tx - 1) / t.tx) * long((ny + t.ty - 1) / t.ty);
+    int lz = interior;
+    while (lz > 16 && cols * ((interior + lz - 1) / lz) < 2048) lz = (lz + 1) / 2;
+    return lz < 1 ? 1 : lz;
+}
+
+// One field set.  `macros` runs the tile-0 kernel that writes every moment.
+template <int Q, class P>
+hipError_t launch_lattice(int tile, bool macros, const LatticeArgs &a, hipStream_t s);
+
+// Ensemble: persistent kernel over all members' tiles; grid = resident blocks.
+template <int Q, class P>
+hipError_t launch_lattice_batch(int tile, bool macros, const JetSlot *jets, int njets, uint32_t total_tiles,
+                                hipStream_t s);
+
+}  // namespace ssmd

@@ -12,9 +12,11 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "../../include/ssmd.h"
-#include "kernels.hpp"
+#define SSMD_STEP_KERNELS 1  // the non-template kernels live in this translation unit
+#include "launch.hpp"
 
 using namespace ssmd;
 
@@ -87,90 +89,10 @@ bool fields_ready(const ssmd_fields *d) {
            d->phi && d->fsx && d->fsy && d->fsz && d->f && d->g && d->f_next && d->g_next;
 }
 
-// Launch shape of the fused lattice kernel.
-constexpr int kTX = 64, kTY = 4;
+int tile_of(const ssmd_params &p) { return (p.tile >= 0 && p.tile < kNumTiles) ? p.tile : 0; }
+int lz_of(const ssmd_params &p) { return p.lz > 0 ? p.lz : auto_lz(int(p.nx), int(p.ny), int(p.nz), tile_of(p)); }
 
-int lattice_lz(const ssmd_params &p) {
-    // enough z-chunks that the grid holds >= ~2048 blocks on large boxes,
-    // without letting the 4-plane halo prologue dominate a chunk
-    const int interior = int(p.nz) - 2;
-    const long cols = long((p.nx + kTX - 1) / kTX) * long((p.ny + kTY - 1) / kTY);
-    int lz = interior;
-    while (lz > 16 && cols * ((interior + lz - 1) / lz) < 2048) lz = (lz + 1) / 2;
-    return lz < 1 ? 1 : lz;
-}
-
-template <int Q, class P, bool MAC>
-int launch_lattice_t(const ssmd_params &p, const KParams &k, const ssmd_fields &d, hipStream_t s) {
-    const int lz = lattice_lz(p);
-    dim3 grid((p.nx + kTX - 1) / kTX, (p.ny + kTY - 1) / kTY, (p.nz - 2 + lz - 1) / lz);
-    dim3 block(kTX, kTY, 1);
-    hipLaunchKernelGGL((k_lattice<Q, P, kTX, kTY, MAC>), grid, block, 0, s, k,
-                       static_cast<const typename P::T *>(d.f), static_cast<typename P::T *>(d.f_next), d.g_next,
-                       d.phi, make_macro(d), lz);
-    return herr(hipGetLastError());
-}
-
-template <int Q, class P>
-int launch_lattice_q(const ssmd_params &p, const KParams &k, const ssmd_fields &d, bool mac, hipStream_t s) {
-    return mac ? launch_lattice_t<Q, P, true>(p, k, d, s) : launch_lattice_t<Q, P, false>(p, k, d, s);
-}
-
-template <int Q, class P>
-int launch_bcs_t(const ssmd_params &p, const KParams &k, const ssmd_fields &d, uint64_t *counter, uint64_t step,
-                 hipStream_t s) {
-    using PT = typename P::T;
-    PT *fo = static_cast<PT *>(d.f_next);
-    const Macro m = make_macro(d);
-    const dim3 b2(64, 4, 1);
-    auto g2 = [&](uint32_t na, uint32_t nb) { return dim3((na + 63) / 64, (nb + 3) / 4, 1); };
-    // which kernel is last (it advances the device step counter)
-    int last = 0;  // 1 outflowY, 2 outflowZ, 3 px, 4 py, 5 pz
-    if (p.outflow_y) last = 1;
-    if (p.outflow_z) last = 2;
-    if (p.wrap_x) last = 3;
-    if (p.wrap_y) last = 4;
-    if (p.wrap_z) last = 5;
-    if (p.water.enabled)
-        hipLaunchKernelGGL((k_inflow<Q, P, 1>), g2(p.nx, p.nz), b2, 0, s, k, fo, d.g_next, d.phi, m, counter, step);
-    if (p.oil.enabled)
-        hipLaunchKernelGGL((k_inflow<Q, P, 2>), g2(p.nx, p.ny), b2, 0, s, k, fo, d.g_next, d.phi, m, counter, step);
-    if (p.outflow_y)
-        hipLaunchKernelGGL((k_outflow<Q, P, 1>), g2(p.nx, p.nz), b2, 0, s, k, fo, d.g_next, d.phi, m, counter,
-                           int(last == 1));
-    if (p.outflow_z)
-        hipLaunchKernelGGL((k_outflow<Q, P, 2>), g2(p.nx, p.ny), b2, 0, s, k, fo, d.g_next, d.phi, m, counter,
-                           int(last == 2));
-    auto per = [&](int axis, int mode, uint32_t na, uint32_t nb, int me) {
-        const int bl = int(last == me);
-        if (axis == 0 && mode == SSMD_WRAP_REFERENCE)
-            hipLaunchKernelGGL((k_periodic<Q, P, 0, 1>), g2(na, nb), b2, 0, s, k, fo, d.g_next, d.phi, m, counter, bl);
-        if (axis == 0 && mode == SSMD_WRAP_TRUE)
-            hipLaunchKernelGGL((k_periodic<Q, P, 0, 2>), g2(na, nb), b2, 0, s, k, fo, d.g_next, d.phi, m, counter, bl);
-        if (axis == 1 && mode == SSMD_WRAP_REFERENCE)
-            hipLaunchKernelGGL((k_periodic<Q, P, 1, 1>), g2(na, nb), b2, 0, s, k, fo, d.g_next, d.phi, m, counter, bl);
-        if (axis == 1 && mode == SSMD_WRAP_TRUE)
-            hipLaunchKernelGGL((k_periodic<Q, P, 1, 2>), g2(na, nb), b2, 0, s, k, fo, d.g_next, d.phi, m, counter, bl);
-        if (axis == 2 && mode == SSMD_WRAP_REFERENCE)
-            hipLaunchKernelGGL((k_periodic<Q, P, 2, 1>), g2(na, nb), b2, 0, s, k, fo, d.g_next, d.phi, m, counter, bl);
-        if (axis == 2 && mode == SSMD_WRAP_TRUE)
-            hipLaunchKernelGGL((k_periodic<Q, P, 2, 2>), g2(na, nb), b2, 0, s, k, fo, d.g_next, d.phi, m, counter, bl);
-    };
-    if (p.wrap_x) per(0, p.wrap_x, p.ny, p.nz, 3);
-    if (p.wrap_y) per(1, p.wrap_y, p.nx, p.nz, 4);
-    if (p.wrap_z) per(2, p.wrap_z, p.nx, p.ny, 5);
-    if (counter && last == 0 && (p.water.enabled || p.oil.enabled)) hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, counter);
-    return herr(hipGetLastError());
-}
-
-int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
-    const KParams k = make_kparams(p);
-    hipLaunchKernelGGL(k_phase, dim3((k.N + 255) / 256), dim3(256), 0, s, k, d.g, d.phi);
-    return herr(hipGetLastError());
-}
-
-// Calls fn.template operator()<Q, P>() for the parameter block's lattice and
-// storage format.
+// Calls fn.template operator()<Q, P>() for the lattice and storage format.
 template <class Fn>
 int dispatch(const ssmd_params &p, Fn &&fn) {
     switch (p.q * 10 + p.pop_format) {
@@ -184,14 +106,98 @@ int dispatch(const ssmd_params &p, Fn &&fn) {
     }
 }
 
-int launch_lattice(const ssmd_params &p, const ssmd_fields &d, uint32_t flags, hipStream_t s) {
+// The face operations of a parameter block, in the order a step runs them.
+int face_kinds(const ssmd_params &p, int *kinds) {
+    int n = 0;
+    if (p.water.enabled) kinds[n++] = kInflowY;
+    if (p.oil.enabled) kinds[n++] = kInflowZ;
+    if (p.outflow_y) kinds[n++] = kOutflowY;
+    if (p.outflow_z) kinds[n++] = kOutflowZ;
+    if (p.wrap_x) kinds[n++] = p.wrap_x == SSMD_WRAP_TRUE ? kPeriodicXTrue : kPeriodicXRef;
+    if (p.wrap_y) kinds[n++] = p.wrap_y == SSMD_WRAP_TRUE ? kPeriodicYTrue : kPeriodicYRef;
+    if (p.wrap_z) kinds[n++] = p.wrap_z == SSMD_WRAP_TRUE ? kPeriodicZTrue : kPeriodicZRef;
+    return n;
+}
+
+bool is_inflow(int kind) { return kind == kInflowY || kind == kInflowZ; }
+
+// Runtime face kind -> template instantiation.
+template <class Fn>
+void face_dispatch(int kind, Fn &&fn) {
+    switch (kind) {
+        case kInflowY: fn.template operator()<kInflowY>(); break;
+        case kInflowZ: fn.template operator()<kInflowZ>(); break;
+        case kOutflowY: fn.template operator()<kOutflowY>(); break;
+        case kOutflowZ: fn.template operator()<kOutflowZ>(); break;
+        case kPeriodicXRef: fn.template operator()<kPeriodicXRef>(); break;
+        case kPeriodicXTrue: fn.template operator()<kPeriodicXTrue>(); break;
+        case kPeriodicYRef: fn.template operator()<kPeriodicYRef>(); break;
+        case kPeriodicYTrue: fn.template operator()<kPeriodicYTrue>(); break;
+        case kPeriodicZRef: fn.template operator()<kPeriodicZRef>(); break;
+        case kPeriodicZTrue: fn.template operator()<kPeriodicZTrue>(); break;
+        default: break;
+    }
+}
+
+int face_axis_host(int kind) {
+    return (kind == kInflowY || kind == kOutflowY || kind == kPeriodicYRef || kind == kPeriodicYTrue)
+               ? 1
+               : ((kind == kPeriodicXRef || kind == kPeriodicXTrue) ? 0 : 2);
+}
+
+int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
     const KParams k = make_kparams(p);
-    return dispatch(p, [&]<int Q, class P>() { return launch_lattice_q<Q, P>(p, k, d, (flags & SSMD_STEP_MACROS) != 0, s); });
+    hipLaunchKernelGGL(k_phase, dim3((k.N + 255) / 256), dim3(256), 0, s, k, d.g, d.phi);
+    return herr(hipGetLastError());
+}
+
+LatticeArgs lattice_args(const ssmd_params &p, const ssmd_fields &d, int lz) {
+    LatticeArgs a{};
+    a.k = make_kparams(p);
+    a.fin = d.f;
+    a.fout = d.f_next;
+    a.gout = d.g_next;
+    a.phi = d.phi;
+    a.mac = make_macro(d);
+    a.lz = lz;
+    return a;
+}
+
+int launch_lattice_sel(const ssmd_params &p, const ssmd_fields &d, uint32_t flags, int tile, int lz, hipStream_t s) {
+    const bool macros = (flags & SSMD_STEP_MACROS) != 0;
+    // the macro-writing kernel is built for tile 0 only; per-cell results do not depend on the tile
+    const int t = macros ? 0 : tile;
+    const LatticeArgs a = lattice_args(p, d, lz > 0 ? lz : auto_lz(int(p.nx), int(p.ny), int(p.nz), t));
+    return dispatch(p, [&]<int Q, class P>() { return herr(launch_lattice<Q, P>(t, macros, a, s)); });
+}
+
+int launch_lattice(const ssmd_params &p, const ssmd_fields &d, uint32_t flags, hipStream_t s) {
+    return launch_lattice_sel(p, d, flags, tile_of(p), p.lz, s);
 }
 
 int launch_boundaries(const ssmd_params &p, const ssmd_fields &d, uint64_t *counter, uint64_t step, hipStream_t s) {
     const KParams k = make_kparams(p);
-    return dispatch(p, [&]<int Q, class P>() { return launch_bcs_t<Q, P>(p, k, d, counter, step, s); });
+    const Macro m = make_macro(d);
+    int kinds[8];
+    const int nk = face_kinds(p, kinds);
+    int last = -1;  // last kind that may advance the step counter
+    for (int i = 0; i < nk; ++i)
+        if (!is_inflow(kinds[i])) last = i;
+    return dispatch(p, [&]<int Q, class P>() {
+        using PT = typename P::T;
+        PT *fo = static_cast<PT *>(d.f_next);
+        for (int i = 0; i < nk; ++i) {
+            int na, nb;
+            face_extent(face_axis_host(kinds[i]), k.nx, k.ny, k.nz, na, nb);
+            const dim3 grid((na + 63) / 64, (nb + 3) / 4);
+            face_dispatch(kinds[i], [&]<int KIND>() {
+                hipLaunchKernelGGL((k_face<Q, P, KIND>), grid, dim3(64, 4), 0, s, k, fo, d.g_next, d.phi, m, counter,
+                                   step, int(i == last));
+            });
+        }
+        if (counter && last < 0 && nk > 0) hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, counter);
+        return herr(hipGetLastError());
+    });
 }
 
 // One step on the buffers as they are in *d (no swap).
@@ -292,6 +298,7 @@ int ssmd_validate(const ssmd_params *p) {
     if (p->wrap_y && (p->water.enabled || p->outflow_y)) return SSMD_EINVAL;
     if (p->wrap_z && (p->oil.enabled || p->outflow_z)) return SSMD_EINVAL;
     if (!(p->visc_water > 0.0) || !(p->visc_oil > 0.0)) return SSMD_EINVAL;
+    if (p->tile < 0 || p->tile >= kNumTiles || p->lz < 0) return SSMD_EINVAL;
     return SSMD_OK;
 }
 
@@ -522,6 +529,331 @@ int ssmd_quantize(int32_t fmt, const float *in, float *out, uint64_t n, void *st
         default: return SSMD_EINVAL;
     }
     return herr(hipGetLastError());
+}
+
+
+// ---------------------------------------------------------------------------
+// Autotuner
+
+int ssmd_tile_count(void) { return kNumTiles; }
+
+int ssmd_tile_shape(int32_t tile, int32_t *tx, int32_t *ty) {
+    if (tile < 0 || tile >= kNumTiles || !tx || !ty) return SSMD_EINVAL;
+    *tx = kTiles[tile].tx;
+    *ty = kTiles[tile].ty;
+    return SSMD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Mean duration (ms) of `reps` launches made by `launch`, after one warm-up.
+template <class F>
+int time_launches(hipStream_t s, int reps, F &&launch, float *ms) {
+    hipEvent_t e0, e1;
+    SSMD_HIP(hipEventCreate(&e0));
+    SSMD_HIP(hipEventCreate(&e1));
+    int rc = launch();
+    if (rc == 0) rc = herr(hipEventRecord(e0, s));
+    for (int r = 0; r < reps && rc == 0; ++r) rc = launch();
+    if (rc == 0) rc = herr(hipEventRecord(e1, s));
+    if (rc == 0) rc = herr(hipEventSynchronize(e1));
+    float t = 0.f;
+    if (rc == 0) rc = herr(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / float(reps > 0 ? reps : 1);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssmd_tune(const ssmd_params *p, const ssmd_fields *d, int32_t reps, void *stream, int32_t *best_tile,
+              int32_t *best_lz) {
+    SSMD_TRY(ssmd_validate(p));
+    if (!fields_ready(d)) return SSMD_ENOTREADY;
+    if (!best_tile || !best_lz || reps < 1) return SSMD_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    SSMD_TRY(launch_phase(*p, *d, s));
+    float best = 1e30f;
+    int bt = 0, bl = 0;
+    const int interior = int(p->nz) - 2;
+    for (int t = 0; t < kNumTiles; ++t) {
+        const int lzs[] = {auto_lz(int(p->nx), int(p->ny), int(p->nz), t), 8, 16, 32, 64};
+        for (int lz : lzs) {
+            if (lz > interior && lz != lzs[0]) continue;
+            float ms = 0.f;
+            SSMD_TRY(time_launches(s, reps, [&]() { return launch_lattice_sel(*p, *d, 0, t, lz, s); }, &ms));
+            if (ms < best) {
+                best = ms;
+                bt = t;
+                bl = lz;
+            }
+        }
+    }
+    *best_tile = bt;
+    *best_lz = bl;
+    return SSMD_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensembles
+
+struct ssmd_ensemble {
+    int n = 0;
+    std::vector<ssmd_params> p;
+    std::vector<ssmd_fields> d;     // current descriptors
+    std::vector<ssmd_fields> orig;  // descriptors at creation (parity 0)
+    int tile = 0;
+    int kinds[8] = {};
+    int nk = 0, last = -1;
+    JetSlot *table[2] = {nullptr, nullptr};
+    uint32_t total_tiles = 0, cell_blocks = 0, face_blocks[3] = {0, 0, 0};
+    int parity = 0;
+    uint64_t *counter = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+void ensemble_drop_graphs(ssmd_ensemble *e) {
+    for (int i = 0; i < 2; ++i) {
+        if (e->exec[i]) (void)hipGraphExecDestroy(e->exec[i]);
+        if (e->graph[i]) (void)hipGraphDestroy(e->graph[i]);
+        e->exec[i] = nullptr;
+        e->graph[i] = nullptr;
+    }
+}
+
+// Host tables for both parities with tile shape `tile`, uploaded to dst[2].
+int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total_tiles) {
+    SSMD_HIP(hipDeviceSynchronize());  // no launch may still read the tables being replaced
+    std::vector<JetSlot> h[2];
+    uint32_t tiles = 0, cells = 0, faces[3] = {0, 0, 0};
+    const TileShape ts = kTiles[tile];
+    for (int j = 0; j < e->n; ++j) {
+        const ssmd_params &pp = e->p[j];
+        const ssmd_fields &o = e->orig[j];
+        JetSlot js{};
+        js.k = make_kparams(pp);
+        js.phi = o.phi;
+        js.mac = make_macro(o);
+        js.tiles_x = int((pp.nx + ts.tx - 1) / ts.tx);
+        js.tiles_y = int((pp.ny + ts.ty - 1) / ts.ty);
+        js.lz = pp.lz > 0 ? pp.lz : auto_lz(int(pp.nx), int(pp.ny), int(pp.nz), tile);
+        const int tz = (int(pp.nz) - 2 + js.lz - 1) / js.lz;
+        js.tile_begin = tiles;
+        tiles += uint32_t(js.tiles_x) * uint32_t(js.tiles_y) * uint32_t(tz);
+        js.cell_block_begin = cells;
+        cells += (js.k.N + 255u) / 256u;
+        for (int ax = 0; ax < 3; ++ax) {
+            int na, nb;
+            face_extent(ax, js.k.nx, js.k.ny, js.k.nz, na, nb);
+            js.face_begin[ax] = faces[ax];
+            faces[ax] += uint32_t((na + 63) / 64) * uint32_t((nb + 3) / 4);
+        }
+        for (int par = 0; par < 2; ++par) {
+            JetSlot t = js;
+            t.f = par == 0 ? o.f : o.f_next;
+            t.f_next = par == 0 ? o.f_next : o.f;
+            t.g = par == 0 ? o.g : o.g_next;
+            t.g_next = par == 0 ? o.g_next : o.g;
+            h[par].push_back(t);
+        }
+    }
+    for (int par = 0; par < 2; ++par)
+        SSMD_HIP(hipMemcpy(dst[par], h[par].data(), sizeof(JetSlot) * size_t(e->n), hipMemcpyHostToDevice));
+    *total_tiles = tiles;
+    e->cell_blocks = cells;
+    for (int ax = 0; ax < 3; ++ax) e->face_blocks[ax] = faces[ax];
+    return SSMD_OK;
+}
+
+int ensemble_launch_step(ssmd_ensemble *e, const JetSlot *tab, int tile, uint32_t total_tiles, uint64_t *counter,
+                         uint64_t step, uint32_t flags, hipStream_t s) {
+    const ssmd_params &p0 = e->p[0];
+    hipLaunchKernelGGL(k_phase_batch, dim3(e->cell_blocks), dim3(256), 0, s, tab, e->n);
+    SSMD_HIP(hipGetLastError());
+    const bool macros = (flags & SSMD_STEP_MACROS) != 0;
+    if (macros && tile != 0) return SSMD_EINVAL;  // tables are laid out for the tile in use
+    SSMD_TRY(dispatch(p0, [&]<int Q, class P>() {
+        return herr(launch_lattice_batch<Q, P>(tile, macros, tab, e->n, total_tiles, s));
+    }));
+    return dispatch(p0, [&]<int Q, class P>() {
+        for (int i = 0; i < e->nk; ++i) {
+            const int ax = face_axis_host(e->kinds[i]);
+            face_dispatch(e->kinds[i], [&]<int KIND>() {
+                hipLaunchKernelGGL((k_face_batch<Q, P, KIND>), dim3(e->face_blocks[ax]), dim3(64, 4), 0, s, tab, e->n,
+                                   counter, step, int(i == e->last));
+            });
+        }
+        if (counter && e->last < 0 && e->nk > 0) hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, counter);
+        return herr(hipGetLastError());
+    });
+}
+
+void ensemble_swap(ssmd_ensemble *e) {
+    for (auto &f : e->d) swap_buffers(&f);
+    e->parity ^= 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssmd_ensemble_create(const ssmd_params *params, const ssmd_fields *fields, int32_t n, ssmd_ensemble **out) {
+    if (!params || !fields || !out || n < 1 || n > 1024) return SSMD_EINVAL;
+    int kinds0[8];
+    const int nk0 = face_kinds(params[0], kinds0);
+    for (int j = 0; j < n; ++j) {
+        SSMD_TRY(ssmd_validate(&params[j]));
+        if (!fields_ready(&fields[j])) return SSMD_ENOTREADY;
+        if (params[j].q != params[0].q || params[j].pop_format != params[0].pop_format) return SSMD_EUNSUPPORTED;
+        int kinds[8];
+        const int nk = face_kinds(params[j], kinds);
+        if (nk != nk0 || std::memcmp(kinds, kinds0, sizeof(int) * size_t(nk)) != 0) return SSMD_EUNSUPPORTED;
+    }
+    ssmd_ensemble *e = new (std::nothrow) ssmd_ensemble();
+    if (!e) return SSMD_EINVAL;
+    e->n = n;
+    e->p.assign(params, params + n);
+    e->d.assign(fields, fields + n);
+    e->orig = e->d;
+    e->nk = nk0;
+    std::memcpy(e->kinds, kinds0, sizeof(kinds0));
+    for (int i = 0; i < nk0; ++i)
+        if (!is_inflow(kinds0[i])) e->last = i;
+    e->tile = params[0].tile;
+    int rc = 0;
+    for (int par = 0; par < 2 && rc == 0; ++par)
+        rc = herr(hipMalloc(reinterpret_cast<void **>(&e->table[par]), sizeof(JetSlot) * size_t(n)));
+    if (rc == 0) rc = herr(hipMalloc(reinterpret_cast<void **>(&e->counter), sizeof(uint64_t)));
+    if (rc == 0) rc = ensemble_tables(e, e->tile, e->table, &e->total_tiles);
+    if (rc != 0) {
+        ssmd_ensemble_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return SSMD_OK;
+}
+
+int ssmd_ensemble_set_tile(ssmd_ensemble *e, int32_t tile) {
+    if (!e || tile < 0 || tile >= kNumTiles) return SSMD_EINVAL;
+    if (tile == e->tile) return SSMD_OK;
+    ensemble_drop_graphs(e);
+    e->tile = tile;
+    return ensemble_tables(e, tile, e->table, &e->total_tiles);
+}
+
+int ssmd_ensemble_step(ssmd_ensemble *e, uint64_t step, uint32_t flags, void *stream) {
+    if (!e) return SSMD_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool macros = (flags & SSMD_STEP_MACROS) != 0;
+    if (macros && e->tile != 0) {
+        // the macro kernel uses tile 0: switch, step, switch back
+        const int keep = e->tile;
+        SSMD_TRY(ssmd_ensemble_set_tile(e, 0));
+        const int rc = ensemble_launch_step(e, e->table[e->parity], 0, e->total_tiles, nullptr, step, flags, s);
+        SSMD_TRY(rc);
+        ensemble_swap(e);
+        SSMD_HIP(hipStreamSynchronize(s));  // the tables are rewritten next
+        return ssmd_ensemble_set_tile(e, keep);
+    }
+    SSMD_TRY(ensemble_launch_step(e, e->table[e->parity], e->tile, e->total_tiles, nullptr, step, flags, s));
+    ensemble_swap(e);
+    return SSMD_OK;
+}
+
+int ssmd_ensemble_graph_launch(ssmd_ensemble *e, uint64_t step0, uint64_t nsteps, void *stream) {
+    if (!e) return SSMD_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!e->exec[0]) {
+        for (int par = 0; par < 2; ++par) {
+            hipError_t err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (err != hipSuccess) return int(err);
+            const int rc = ensemble_launch_step(e, e->table[par], e->tile, e->total_tiles, e->counter, 0, 0, s);
+            err = hipStreamEndCapture(s, &e->graph[par]);
+            if (rc != 0 || err != hipSuccess) {
+                ensemble_drop_graphs(e);
+                return rc != 0 ? rc : int(err);
+            }
+            err = hipGraphInstantiate(&e->exec[par], e->graph[par], nullptr, nullptr, 0);
+            if (err != hipSuccess) {
+                ensemble_drop_graphs(e);
+                return int(err);
+            }
+        }
+    }
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, s, e->counter, step0);
+    SSMD_HIP(hipGetLastError());
+    for (uint64_t i = 0; i < nsteps; ++i) {
+        SSMD_HIP(hipGraphLaunch(e->exec[e->parity], s));
+        ensemble_swap(e);
+    }
+    return SSMD_OK;
+}
+
+int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *best_tile) {
+    if (!e || !best_tile || reps < 1) return SSMD_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    JetSlot *tmp[2] = {nullptr, nullptr};
+    int rc = 0;
+    for (int par = 0; par < 2 && rc == 0; ++par)
+        rc = herr(hipMalloc(reinterpret_cast<void **>(&tmp[par]), sizeof(JetSlot) * size_t(e->n)));
+    float best = 1e30f;
+    int bt = e->tile;
+    const ssmd_params &p0 = e->p[0];
+    if (rc == 0) {
+        hipLaunchKernelGGL(k_phase_batch, dim3(e->cell_blocks), dim3(256), 0, s, e->table[e->parity], e->n);
+        rc = herr(hipGetLastError());
+    }
+    for (int t = 0; t < kNumTiles && rc == 0; ++t) {
+        uint32_t total = 0;
+        rc = ensemble_tables(e, t, tmp, &total);
+        if (rc != 0) break;
+        float ms = 0.f;
+        rc = time_launches(s, reps, [&]() {
+            return dispatch(p0, [&]<int Q, class P>() {
+                return herr(launch_lattice_batch<Q, P>(t, false, tmp[e->parity], e->n, total, s));
+            });
+        }, &ms);
+        if (rc == 0 && ms < best) {
+            best = ms;
+            bt = t;
+        }
+    }
+    for (int par = 0; par < 2; ++par)
+        if (tmp[par]) (void)hipFree(tmp[par]);
+    if (rc != 0) return rc;
+    // restore the face/cell block counts of the tile in use, then switch
+    uint32_t total = 0;
+    SSMD_TRY(ensemble_tables(e, e->tile, e->table, &total));
+    e->total_tiles = total;
+    SSMD_TRY(ssmd_ensemble_set_tile(e, bt));
+    *best_tile = bt;
+    return SSMD_OK;
+}
+
+int ssmd_ensemble_fields(const ssmd_ensemble *e, int32_t member, ssmd_fields *out) {
+    if (!e || !out || member < 0 || member >= e->n) return SSMD_EINVAL;
+    *out = e->d[size_t(member)];
+    return SSMD_OK;
+}
+
+int ssmd_ensemble_destroy(ssmd_ensemble *e) {
+    if (!e) return SSMD_OK;
+    ensemble_drop_graphs(e);
+    for (int par = 0; par < 2; ++par)
+        if (e->table[par]) (void)hipFree(e->table[par]);
+    if (e->counter) (void)hipFree(e->counter);
+    delete e;
+    return SSMD_OK;
 }
 
 }  // extern "C"

@@ -104,6 +104,17 @@ class Simulation:
                   "ssmd_graph_launch")
         self.step_index += n
 
+    def tune(self, reps: int = 20) -> tuple[int, int]:
+        """Autotune the lattice tile shape and z-chunk; the choice is kept in self.p."""
+        t, lz = C.c_int32(), C.c_int32()
+        abi.check(self.lib.ssmd_tune(C.byref(self.p), C.byref(self.d), reps, self.stream_handle, C.byref(t),
+                                     C.byref(lz)), "ssmd_tune")
+        self.p.tile, self.p.lz = t.value, lz.value
+        if self._graph is not None:  # graphs hold the old launch shape
+            self.lib.ssmd_graph_destroy(self._graph)
+            self._graph = None
+        return t.value, lz.value
+
     def close(self) -> None:
         if self._graph is not None:
             self.lib.ssmd_graph_destroy(self._graph)
@@ -192,3 +203,79 @@ def arrival_to_source(arr: np.ndarray) -> np.ndarray:
         sx = slice(max(-cx, 0), nx + min(-cx, 0))
         out[i][sz, sy, sx] = arr[i][az, ay, ax]
     return out
+
+
+class Ensemble:
+    """Independent jets advanced together by single batched launches
+    (persistent lattice kernel, batched phase and face kernels)."""
+
+    def __init__(self, params_list, device: str | torch.device = "cuda:0"):
+        self.lib = abi.load()
+        self.members = [Simulation(p, device) for p in params_list]
+        self.stream = torch.cuda.Stream(torch.device(device))
+        for m in self.members:
+            m.stream = self.stream
+        self.n = len(self.members)
+        self._p = (abi.Params * self.n)(*[m.p for m in self.members])
+        self._h = None
+        self.step_index = 0
+
+    @property
+    def stream_handle(self) -> int:
+        return self.stream.cuda_stream
+
+    def init(self) -> None:
+        for m in self.members:
+            m.init()
+        torch.cuda.synchronize()
+        if self._h is None:
+            d = (abi.Fields * self.n)(*[m.d for m in self.members])
+            h = C.c_void_p()
+            abi.check(self.lib.ssmd_ensemble_create(self._p, d, self.n, C.byref(h)), "ssmd_ensemble_create")
+            self._h = h
+        self.step_index = 0
+
+    def _sync_members(self) -> None:
+        for j, m in enumerate(self.members):
+            abi.check(self.lib.ssmd_ensemble_fields(self._h, j, C.byref(m.d)), "ssmd_ensemble_fields")
+            m.step_index = self.step_index
+
+    def step(self, n: int = 1, macros: bool = False) -> None:
+        for i in range(n):
+            flags = abi.STEP_MACROS if (macros and i == n - 1) else 0
+            abi.check(self.lib.ssmd_ensemble_step(self._h, self.step_index, flags, self.stream_handle),
+                      "ssmd_ensemble_step")
+            self.step_index += 1
+        self._sync_members()
+
+    def graph_run(self, n: int) -> None:
+        abi.check(self.lib.ssmd_ensemble_graph_launch(self._h, self.step_index, n, self.stream_handle),
+                  "ssmd_ensemble_graph_launch")
+        self.step_index += n
+        self._sync_members()
+
+    def tune(self, reps: int = 10) -> int:
+        t = C.c_int32()
+        abi.check(self.lib.ssmd_ensemble_tune(self._h, reps, self.stream_handle, C.byref(t)), "ssmd_ensemble_tune")
+        return t.value
+
+    def set_tile(self, tile: int) -> None:
+        abi.check(self.lib.ssmd_ensemble_set_tile(self._h, tile), "ssmd_ensemble_set_tile")
+
+    def synchronize(self) -> None:
+        self.stream.synchronize()
+
+    @property
+    def cells(self) -> int:
+        return sum(m.p.cells for m in self.members)
+
+    def close(self) -> None:
+        if self._h is not None:
+            self.lib.ssmd_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
