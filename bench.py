@@ -35,7 +35,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measure(case, steps: int, warmup: int, kernel_steps: int):
+def measure(case, steps: int, warmup: int, kernel_steps: int, tune: bool = True):
     import torch
 
     from ssmdlbm_amd import cases
@@ -45,6 +45,10 @@ def measure(case, steps: int, warmup: int, kernel_steps: int):
     sim = Simulation(p, "cuda:0")
     sim.init()
     interior = (p.nx - 2) * (p.ny - 2) * (p.nz - 2)
+    tile = (int(sim.p.tile), int(sim.p.lz))
+    if tune:  # the autotuner picks the lattice tile shape and z-chunk on this state
+        sim.step(2)
+        tile = sim.tune(reps=10)
     if warmup:
         sim.graph_run(warmup)
     torch.cuda.synchronize()
@@ -74,6 +78,7 @@ def measure(case, steps: int, warmup: int, kernel_steps: int):
     del sim
     torch.cuda.empty_cache()
     return {
+        "tile": list(tile),
         "mlups": mlups,
         "ms_per_step": wall / steps * 1e3,
         "kernel_ms": k_ms,
@@ -83,6 +88,57 @@ def measure(case, steps: int, warmup: int, kernel_steps: int):
         "interior_cells": interior,
         "step_bytes_per_cell": step_bytes,
         "step_frac": mlups * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9),
+    }
+
+
+def measure_ensemble(steps: int, warmup: int, n_jets: int = 32):
+    """The batching form: a seeded ensemble of jets advanced by one launch per
+    kernel kind per step (graph-replayed, autotuned tile), against the same
+    jets stepped one after another with their own graphs."""
+    import torch
+
+    from ssmdlbm_amd import cases
+    from ssmdlbm_amd.solver import Ensemble, Simulation
+
+    members = cases.ensemble(n_jets, seed=4)
+    ens = Ensemble([c.params() for c in members], "cuda:0")
+    ens.init()
+    ens.step(2)
+    tile = ens.tune(reps=5)
+    ens.graph_run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.graph_run(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = ens.cells
+    ens.close()
+    del ens
+    torch.cuda.empty_cache()
+    sims = [Simulation(c.params(), "cuda:0") for c in members]
+    for s in sims:
+        s.init()
+        s.graph_run(max(warmup // 10, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in sims:
+            s.graph_run(1)
+    torch.cuda.synchronize()
+    wall_seq = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    del sims
+    torch.cuda.empty_cache()
+    return {
+        "jets": n_jets,
+        "cells": cells,
+        "tile": tile,
+        "steps": steps,
+        "mlups": round(cells * steps / wall / 1e6, 2),
+        "ms_per_step": round(wall / steps * 1e3, 4),
+        "sequential_mlups": round(cells * steps / wall_seq / 1e6, 2),
+        "batched_over_sequential": round(wall_seq / wall, 3),
     }
 
 
@@ -153,6 +209,7 @@ def main() -> int:
     ap.add_argument("--form-steps", type=int, default=300)
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16-vs-FP32 storage measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default lattice tile shape")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling mode for rocprofv3 --pmc: N eager steps of the workload, then a plain "
                          "1 GiB device copy (the byte-count reference), no timing output")
@@ -175,19 +232,21 @@ def main() -> int:
     if args.profile_steps:
         return profile_mode(case, args.profile_steps)
     log(f"workload {case.name}: warmup {args.warmup}, timed {args.steps}")
-    main_m = measure(case, args.steps, args.warmup, args.kernel_steps)
+    main_m = measure(case, args.steps, args.warmup, args.kernel_steps, tune=not args.no_tune)
     log(f"main: {main_m}")
 
     forms = None
     if not args.no_forms:
         forms = {}
         for c in (cases.JET_D3Q19_FP32_LARGE, cases.JET_D3Q19_FP16):
-            m = measure(c, args.form_steps, 50, min(args.kernel_steps, 100))
+            m = measure(c, args.form_steps, 50, min(args.kernel_steps, 100), tune=not args.no_tune)
             forms[c.name] = {k: round(v, 4) if isinstance(v, float) else v for k, v in m.items()}
             log(f"form {c.name}: {m}")
         a = forms[cases.JET_D3Q19_FP16.name]["mlups"]
         b = forms[cases.JET_D3Q19_FP32_LARGE.name]["mlups"]
         forms["fp16_over_fp32_mlups"] = round(a / b, 4)
+        forms["ensemble_32_jets_d3q19"] = measure_ensemble(args.form_steps, 50)
+        log(f"ensemble: {forms['ensemble_32_jets_d3q19']}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -217,6 +276,7 @@ def main() -> int:
             "tau": case.tau, "tau_g": case.tau_g, "sigma": case.sigma, "inlet_diameter": case.diameter,
             "u_inlet": case.u, "perturb": case.perturb,
             "timed_region": "HIP graph replay, one graph per step (phase, fused lattice, boundary kernels)",
+            "lattice_tile": main_m["tile"],
             "mlups_cells": "all cells incl. ghost layer (reference convention)",
         },
         "roofline": {

@@ -39,6 +39,7 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#include <xmmintrin.h>
 
 namespace {
 
@@ -267,7 +268,18 @@ struct Oracle final : Base {
         gamma = T(p.gamma);
     }
 
-    void set_threads(int n) override { threads = n; }
+    void set_threads(int n) override {
+        threads = n;
+        flush_denormals();
+    }
+    // FP32 mode flushes denormals to zero (x86 MXCSR FTZ|DAZ) in every worker:
+    // far-field populations decay into the denormal range, where x86 is slow.
+    void flush_denormals() const {
+        if (sizeof(T) != 4) return;
+        _mm_setcsr(_mm_getcsr() | 0x8040u);
+#pragma omp parallel num_threads(nthreads())
+        { _mm_setcsr(_mm_getcsr() | 0x8040u); }
+    }
     int nthreads() const {
 #ifdef _OPENMP
         return threads > 0 ? threads : omp_get_max_threads();
@@ -284,6 +296,7 @@ struct Oracle final : Base {
 
     // -- init (initialConditions.cu:33-140) --------------------------------
     void init() override {
+        flush_denormals();
         for (auto &v : fld) std::fill(v.begin(), v.end(), T(0));
         std::fill(f.begin(), f.end(), T(0));
         std::fill(g.begin(), g.end(), T(0));

@@ -17,6 +17,7 @@
 //   boundary kernels in reference order (K5-K9, + periodic y / z)
 #pragma once
 
+#include "../../include/ssmd.h"
 #include "lattice.hpp"
 
 namespace ssmd {
@@ -36,6 +37,7 @@ struct KParams {
     float perturb;
     uint64_t seed;
     JetK water, oil;
+    int wrap_x, wrap_y, wrap_z;  // SSMD_WRAP_*
 };
 
 struct Macro {
@@ -60,14 +62,32 @@ __device__ __forceinline__ void phase_cell(const KParams &k, const float *__rest
     const int y = int(r % uint32_t(k.ny));
     const int z = int(r / uint32_t(k.ny));
     if (!is_interior(k, x, y, z)) return;
+    // source slots across a periodic face are read from their periodic image
+    int sxp = 0, sxm = 0, syp = 0, sym = 0, szp = 0, szm = 0;
+    if (k.wrap_x) {
+        const int sh = k.wrap_x == SSMD_WRAP_TRUE ? k.nx - 2 : k.nx - 3;
+        if (x == 1) sxp = sh;
+        if (x == k.nx - 2) sxm = -sh;
+    }
+    if (k.wrap_y) {
+        const int sh = (k.wrap_y == SSMD_WRAP_TRUE ? k.ny - 2 : k.ny - 3) * k.nx;
+        if (y == 1) syp = sh;
+        if (y == k.ny - 2) sym = -sh;
+    }
+    if (k.wrap_z) {
+        const int sh = (k.wrap_z == SSMD_WRAP_TRUE ? k.nz - 2 : k.nz - 3) * int(k.sxy);
+        if (z == 1) szp = sh;
+        if (z == k.nz - 2) szm = -sh;
+    }
     const size_t N = k.N;
+    const int ci = int(c);
     float s = g[c];
-    s += g[N + c - 1u];
-    s += g[2 * N + c + 1u];
-    s += g[3 * N + c - uint32_t(k.nx)];
-    s += g[4 * N + c + uint32_t(k.nx)];
-    s += g[5 * N + c - k.sxy];
-    s += g[6 * N + c + k.sxy];
+    s += g[N + uint32_t(ci - 1 + sxp)];
+    s += g[2 * N + uint32_t(ci + 1 + sxm)];
+    s += g[3 * N + uint32_t(ci - k.nx + syp)];
+    s += g[4 * N + uint32_t(ci + k.nx + sym)];
+    s += g[5 * N + uint32_t(ci - int(k.sxy) + szp)];
+    s += g[6 * N + uint32_t(ci + int(k.sxy) + szm)];
     phi[c] = s;
 }
 
@@ -80,47 +100,87 @@ __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restric
 
 // --------------------------------------------------------------------------
 // Fused lattice kernel.  A block of TX x TY threads owns an x-y tile and
-// marches over a chunk of z planes.  LDS rings (5 phi planes with a 2-cell
-// halo, masked and raw; 3 planes of masked normals with a 1-cell halo) give
-// every thread the radius-2 neighbourhood the force and normal stencils need.
+// marches over a chunk of z planes.  LDS rings hold 5 planes of phi-hat
+// (phi on interior cells, 0 on the ghost layer and outside; phaseField.cuh:
+// 125-142) with a 2-cell halo and 3 planes of the masked normals with a
+// 1-cell halo: every thread gets the radius-2 neighbourhood of the force.
 template <int TX, int TY>
 struct LatticeSmem {
     static constexpr int HX = TX + 4, HY = TY + 4, HP = HX * HY;
     static constexpr int NXl = TX + 2, NYl = TY + 2, NP = NXl * NYl;
-    float m[5][HP];   // phi-hat: phi on interior cells, 0 on ghosts / outside
-    float r[5][HP];   // phi as stored (0 outside the domain)
+    float m[5][HP];   // phi-hat ring
     float4 n[3][NP];  // masked normal (x,y,z) and |grad phi-hat|
 };
 
+// Byte offset of the periodic image of a ghost source slot, for the
+// directions that pull across a periodic face (the reference's periodic
+// copies, BoundaryConditions.cuh:305-332, and the TRUE wrap, folded into the
+// addresses: the copy kernels never run).  `lo` = cell index 1 (its source
+// slot 0 is a ghost for c > 0), `hi` = cell n-2 (source n-1 for c < 0).
+__device__ __forceinline__ int wrap_shift(int mode, int n) { return mode == SSMD_WRAP_TRUE ? n - 2 : n - 3; }
+
+// Correctly rounded sqrt/division are not needed here: one-ulp hardware ops.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // One tile: cells x0..x0+TX-1, y0..y0+TY-1, planes zb..ze-1 (threadIdx = (tx, ty)).
+// TX is a multiple of 64, so a wavefront covers part of one x row: y is
+// uniform across a wavefront and z across the block.
 template <int Q, class P, int TX, int TY, bool MACROS>
 __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P::T *__restrict__ fin,
                                              typename P::T *__restrict__ fout, float *__restrict__ gout,
                                              const float *__restrict__ phi, const Macro &mac, int x0, int y0,
                                              int zb, int ze, LatticeSmem<TX, TY> &sm) {
+    static_assert(TX % 64 == 0, "a wavefront must not span two rows");
     using S = LatticeSmem<TX, TY>;
+    using PT = typename P::T;
     constexpr int HX = S::HX, HP = S::HP, NXl = S::NXl, NP = S::NP;
     constexpr int NT = TX * TY;
+    constexpr int EPT = (HP + NT - 1) / NT;  // phi-plane elements per thread
     auto &s_m = sm.m;
-    auto &s_r = sm.r;
     auto &s_n = sm.n;
 
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int tx = threadIdx.x, tid = tx + TX * int(threadIdx.y);
+    // a wavefront lies in one row: make that visible to the compiler
+    const int ty = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x >= 1 && x <= k.nx - 2 && y >= 1 && y <= k.ny - 2;
-    const size_t N = k.N;
+    const size_t N0 = k.N;
+    const uint32_t sxy = k.sxy;
 
-    auto load_plane = [&](int p) {
+    // per-lane x wrap of the pulled source slot (bytes), uniform y wrap (cells)
+    int xsp = 0, xsm = 0;
+    if (k.wrap_x) {
+        const int sh = wrap_shift(k.wrap_x, k.nx);
+        if (x == 1) xsp = sh;
+        if (x == k.nx - 2) xsm = -sh;
+    }
+    long ysp = 0, ysm = 0;
+    if (k.wrap_y) {
+        const long sh = long(wrap_shift(k.wrap_y, k.ny)) * k.nx;
+        if (y == 1) ysp = sh;
+        if (y == k.ny - 2) ysm = -sh;
+    }
+
+    // phi plane p: element e of the (TX+4)(TY+4) halo plane
+    auto phi_at = [&](int e, int p) -> float {
+        const int gx = x0 + e % HX - 2, gy = y0 + e / HX - 2;
+        const bool ii = p >= 1 && p <= k.nz - 2 && gx >= 1 && gx <= k.nx - 2 && gy >= 1 && gy <= k.ny - 2;
+        return ii ? phi[cell_id(k, gx, gy, p)] : 0.f;
+    };
+    auto fetch_plane = [&](int p, float (&r)[EPT]) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * NT;
+            r[q] = e < HP ? phi_at(e, p) : 0.f;
+        }
+    };
+    auto store_plane = [&](int p, const float (&r)[EPT]) {
         const int sl = (p + 5) % 5;
-        const bool pin = p >= 0 && p < k.nz;
-        const bool pint = p >= 1 && p <= k.nz - 2;
-        for (int e = tid; e < HP; e += NT) {
-            const int gx = x0 + e % HX - 2, gy = y0 + e / HX - 2;
-            const bool in = pin && gx >= 0 && gx < k.nx && gy >= 0 && gy < k.ny;
-            const float v = in ? phi[cell_id(k, gx, gy, p)] : 0.f;
-            const bool ii = pint && gx >= 1 && gx <= k.nx - 2 && gy >= 1 && gy <= k.ny - 2;
-            s_r[sl][e] = v;
-            s_m[sl][e] = ii ? v : 0.f;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * NT;
+            if (e < HP) s_m[sl][e] = r[q];
         }
     };
     auto normals = [&](int p) {
@@ -134,40 +194,70 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             };
             float gx, gy, gz;
             iso_grad<Q>(at, gx, gy, gz);
-            const float ind = __fsqrt_rn(gx * gx + gy * gy + gz * gz);
-            const float inv = 1.0f / (ind + 1e-9f);
+            const float ind = fast_sqrt(gx * gx + gy * gy + gz * gz);
+            const float inv = fast_rcp(ind + 1e-9f);
             s_n[sl][e] = make_float4(gx * inv, gy * inv, gz * inv, ind);
         }
     };
 
-    for (int p = zb - 2; p <= zb + 1; ++p) load_plane(p);
+    {
+        float r[EPT];
+        for (int p = zb - 2; p <= zb + 1; ++p) {
+            fetch_plane(p, r);
+            store_plane(p, r);
+        }
+    }
     __syncthreads();
     normals(zb - 1);
     normals(zb);
+    float nxt[EPT];
+    fetch_plane(zb + 2, nxt);  // one plane ahead, in registers
 
     for (int z = zb; z < ze; ++z) {
+        // (1) the prefetched phi plane z+2 into LDS; start fetching z+3
+        store_plane(z + 2, nxt);
+        if (z + 1 < ze) fetch_plane(z + 3, nxt);
+
+        // (2) pull this plane's populations.  The plane stride is made opaque
+        //     per iteration so the 2Q uniform plane offsets are recomputed on
+        //     the scalar unit instead of being held (and spilled) across the loop.
+        size_t N = N0;
+        asm volatile("" : "+s"(N));
         const uint32_t c = cell_id(k, x, y, z);
+        long zsp = 0, zsm = 0;
+        if (k.wrap_z) {
+            const long sh = long(wrap_shift(k.wrap_z, k.nz)) * long(sxy);
+            if (z == 1) zsp = sh;
+            if (z == k.nz - 2) zsm = -sh;
+        }
         float f[Q];
         if (active) {
+            const PT *p0 = fin + c;
+            const PT *pp = p0 + xsp;  // sources of c_x > 0 directions
+            const PT *pm = p0 + xsm;  // sources of c_x < 0 directions
             sfor<0, Q>([&](auto I) {
                 constexpr int i = I;
                 constexpr Dir d = kDir[i];
-                const uint32_t src = c - uint32_t(d.x) - uint32_t(d.y) * uint32_t(k.nx) - uint32_t(d.z) * k.sxy;
-                f[i] = P::load(fin[size_t(i) * N + src]);
+                long shift = long(i) * long(N) - long(d.x) - long(d.y) * k.nx - long(d.z) * long(sxy);
+                if constexpr (d.y > 0) shift += ysp;
+                if constexpr (d.y < 0) shift += ysm;
+                if constexpr (d.z > 0) shift += zsp;
+                if constexpr (d.z < 0) shift += zsm;
+                const PT *b = d.x > 0 ? pp : (d.x < 0 ? pm : p0);
+                f[i] = P::load(b[shift]);
             });
         }
-        load_plane(z + 2);
         __syncthreads();
         normals(z + 1);
         __syncthreads();
         if (!active) continue;
 
-        // surface-tension force from the masked normals (phaseField.cuh:246-293)
+        // (3) surface-tension force from the masked normals (phaseField.cuh:246-293)
         const int nzm = (z + 2) % 3, nz0 = z % 3, nzp = (z + 1) % 3;
         const int ncen = (ty + 1) * NXl + tx + 1;
         float kx, ky, kz;
+        const float4 nc = s_n[nz0][ncen];
         {
-            float sx = 0.f, sy = 0.f, sz = 0.f;
             auto nat = [&](int dx, int dy, int dz) {
                 const int s = dz < 0 ? nzm : (dz > 0 ? nzp : nz0);
                 return s_n[s][ncen + dy * NXl + dx];
@@ -183,50 +273,81 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
                 if constexpr (d.z != 0) acc[cls][2] += d.z > 0 ? a.z - b.z : b.z - a.z;
             });
             constexpr float w1 = weight<Q>(1), w2 = weight<Q>(7), w3 = Q == 27 ? weight<Q>(19) : 0.f;
-            sx = w1 * acc[0][0] + w2 * acc[1][0];
-            sy = w1 * acc[0][1] + w2 * acc[1][1];
-            sz = w1 * acc[0][2] + w2 * acc[1][2];
+            float sx = w1 * acc[0][0] + w2 * acc[1][0];
+            float sy = w1 * acc[0][1] + w2 * acc[1][1];
+            float sz = w1 * acc[0][2] + w2 * acc[1][2];
             if constexpr (Q == 27) {
                 sx += w3 * acc[2][0];
                 sy += w3 * acc[2][1];
                 sz += w3 * acc[2][2];
             }
             const float curvature = 3.0f * (sx + sy + sz);
-            const float4 nc = s_n[nz0][ncen];
             const float st = -k.sigma * curvature * nc.w;
             kx = st * nc.x;
             ky = st * nc.y;
             kz = st * nc.z;
         }
 
-        // normals of the stored phi for the phase update (lbm.cu:157-224)
-        const int rzm = (z + 4) % 5, rz0 = z % 5, rzp = (z + 1) % 5;
-        const int rcen = (ty + 2) * HX + tx + 2;
-        float nx, ny, nz, ph;
-        {
-            auto at = [&](int dx, int dy, int dz) {
-                const int s = dz < 0 ? rzm : (dz > 0 ? rzp : rz0);
-                return s_r[s][rcen + dy * HX + dx];
+        // (4) normal of the stored phi for the phase update (lbm.cu:157-224).
+        //     It equals the masked normal unless the stencil reaches the ghost
+        //     layer; there the ghost values are added back.
+        float nx = nc.x, ny = nc.y, nz = nc.z;
+        const float ph = s_m[z % 5][(ty + 2) * HX + tx + 2];
+        const bool fx_lo = x == 1, fx_hi = x == k.nx - 2, fy_lo = y == 1, fy_hi = y == k.ny - 2;
+        const bool fz_lo = z == 1, fz_hi = z == k.nz - 2;
+        if (fx_lo || fx_hi || fy_lo || fy_hi || fz_lo || fz_hi) {
+            const float g = nc.w + 1e-9f;
+            float gx = nc.x * g, gy = nc.y * g, gz = nc.z * g;
+            // add 3 w_i c_i phi(x + c_i) for every neighbour on the ghost layer,
+            // face by face; a neighbour on two ghost faces is taken by the first
+            auto face_sum = [&](auto axis_c, auto sign_c, bool on) {
+                constexpr int AX = decltype(axis_c)::value, SG = decltype(sign_c)::value;
+                if (!on) return;
+                sfor<1, Q>([&](auto I) {
+                    constexpr int i = I;
+                    constexpr Dir d = kDir[i];
+                    constexpr int cv[3] = {d.x, d.y, d.z};
+                    if constexpr (cv[AX] == SG) {
+                        bool taken = false;
+                        if constexpr (AX >= 1 && d.x != 0) taken = taken || (d.x < 0 ? fx_lo : fx_hi);
+                        if constexpr (AX >= 2 && d.y != 0) taken = taken || (d.y < 0 ? fy_lo : fy_hi);
+                        const float v = 3.0f * weight<Q>(i) * phi[cell_id(k, x + d.x, y + d.y, z + d.z)];
+                        const float vv = taken ? 0.f : v;
+                        if constexpr (d.x != 0) gx += float(d.x) * vv;
+                        if constexpr (d.y != 0) gy += float(d.y) * vv;
+                        if constexpr (d.z != 0) gz += float(d.z) * vv;
+                    }
+                });
             };
-            float gx, gy, gz;
-            iso_grad<Q>(at, gx, gy, gz);
-            const float ind = __fsqrt_rn(gx * gx + gy * gy + gz * gz);
-            const float inv = 1.0f / (ind + 1e-9f);
+            using I0 = std::integral_constant<int, 0>;
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            using Pm = std::integral_constant<int, -1>;
+            using Pp = std::integral_constant<int, 1>;
+            face_sum(I0{}, Pm{}, fx_lo);
+            face_sum(I0{}, Pp{}, fx_hi);
+            face_sum(I1{}, Pm{}, fy_lo);
+            face_sum(I1{}, Pp{}, fy_hi);
+            face_sum(I2{}, Pm{}, fz_lo);
+            face_sum(I2{}, Pp{}, fz_hi);
+            const float ind = fast_sqrt(gx * gx + gy * gy + gz * gz);
+            const float inv = fast_rcp(ind + 1e-9f);
             nx = gx * inv;
             ny = gy * inv;
             nz = gz * inv;
-            ph = s_r[rz0][rcen];
         }
 
+        // (5) moments, regularized collision, phase populations
         const Cell m = moments<Q>(f, kx, ky, kz);
         const float nu = __builtin_fmaf(ph, k.visc_diff, k.visc_water);
-        const float omega = 1.0f / (0.5f + 3.0f * nu);
-        const float omco = 1.0f - omega;
-        collide<Q>(m, omco, kx, ky, kz, [&](int i, float v) { fout[size_t(i) * N + c] = P::store(v); });
+        const float omco = 1.0f - fast_rcp(0.5f + 3.0f * nu);
+        PT *fo = fout + c;
+        collide<Q>(m, omco, kx, ky, kz, [&](int i, float v) { fo[size_t(i) * N] = P::store(v); });
         const float sharp = k.gamma * ph * (1.0f - ph);
-        phase_collide(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) { gout[size_t(kk) * N + c] = v; });
+        float *go = gout + c;
+        phase_collide(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) { go[size_t(kk) * N] = v; });
 
-        const bool face = x == 1 || x == k.nx - 2 || y == 1 || y == k.ny - 2 || z == 1 || z == k.nz - 2;
+        const bool face = fx_lo || fx_hi || fy_lo || fy_hi || fz_lo || fz_hi;
         if (MACROS || face) {
             mac.rho[c] = m.rho;
             mac.ux[c] = m.ux;
@@ -465,10 +586,13 @@ __device__ __forceinline__ void outflow_face(const KParams &k, typename P::T *__
 
 // periodicX (BoundaryConditions.cuh:295-349) and its mirrors on y and z.
 // Face coordinates (a, b) run over the two other axes in increasing order.
-template <int Q, class P, int AXIS, int MODE>
-__device__ __forceinline__ void periodic_face(const KParams &k, typename P::T *__restrict__ fout,
-                                              float *__restrict__ gout, float *__restrict__ phi, const Macro &mac,
-                                              int a, int b) {
+template <int AXIS>
+__device__ __forceinline__ void periodic_face(const KParams &k, float *__restrict__ phi, const Macro &mac, int a, int b,
+                                              bool with_u) {
+    // periodicX's ghost wrap of phi and u (BoundaryConditions.cuh:334-348) and
+    // its mirrors.  The population copies of the same kernel are folded into
+    // the pull addresses of k_phase / k_lattice (wrap_shift), so the slots
+    // they would write are never read and are not written here.
     constexpr int A1 = AXIS == 0 ? 1 : 0, A2 = AXIS == 2 ? 1 : 2;
     const int n[3] = {k.nx, k.ny, k.nz};
     if (a < n[A1] && b < n[A2]) {
@@ -480,47 +604,17 @@ __device__ __forceinline__ void periodic_face(const KParams &k, typename P::T *_
             r[A2] = bb;
             return cell_id(k, r[0], r[1], r[2]);
         };
-        sfor<1, Q>([&](auto I) {
-            constexpr int i = I;
-            constexpr Dir c = kDir[i];
-            constexpr int cv[3] = {c.x, c.y, c.z};
-            constexpr int ca = cv[AXIS], c1 = cv[A1], c2 = cv[A2];
-            if constexpr (ca != 0) {
-                if constexpr (MODE == 1) {
-                    // arrival planes 1 <-> n-2 exchange, expressed on source slots
-                    const int sa = a - c1, sb = b - c2;
-                    if (sa >= 0 && sa < n[A1] && sb >= 0 && sb < n[A2]) {
-                        const uint32_t dst = ca > 0 ? at(0, sa, sb) : at(nn - 1, sa, sb);
-                        const uint32_t src = ca > 0 ? at(nn - 3, sa, sb) : at(2, sa, sb);
-                        fout[size_t(i) * k.N + dst] = fout[size_t(i) * k.N + src];
-                    }
-                } else {
-                    const uint32_t dst = ca > 0 ? at(0, a, b) : at(nn - 1, a, b);
-                    const uint32_t src = ca > 0 ? at(nn - 2, a, b) : at(1, a, b);
-                    fout[size_t(i) * k.N + dst] = fout[size_t(i) * k.N + src];
-                }
-            }
-        });
-        // D3Q7: the two directions along AXIS
-        {
-            const int kp = 1 + 2 * AXIS, km = 2 + 2 * AXIS;
-            if (MODE == 1) {
-                gout[size_t(kp) * k.N + at(0, a, b)] = gout[size_t(kp) * k.N + at(nn - 3, a, b)];
-                gout[size_t(km) * k.N + at(nn - 1, a, b)] = gout[size_t(km) * k.N + at(2, a, b)];
-            } else {
-                gout[size_t(kp) * k.N + at(0, a, b)] = gout[size_t(kp) * k.N + at(nn - 2, a, b)];
-                gout[size_t(km) * k.N + at(nn - 1, a, b)] = gout[size_t(km) * k.N + at(1, a, b)];
-            }
-        }
         const uint32_t bL = at(1, a, b), bR = at(nn - 2, a, b), gL = at(0, a, b), gR = at(nn - 1, a, b);
         phi[gL] = phi[bR];
         phi[gR] = phi[bL];
-        mac.ux[gL] = mac.ux[bR];
-        mac.ux[gR] = mac.ux[bL];
-        mac.uy[gL] = mac.uy[bR];
-        mac.uy[gR] = mac.uy[bL];
-        mac.uz[gL] = mac.uz[bR];
-        mac.uz[gR] = mac.uz[bL];
+        if (with_u) {  // u on the ghost layer is read by no kernel; kept current on macro steps
+            mac.ux[gL] = mac.ux[bR];
+            mac.ux[gR] = mac.ux[bL];
+            mac.uy[gL] = mac.uy[bR];
+            mac.uy[gR] = mac.uy[bL];
+            mac.uz[gL] = mac.uz[bR];
+            mac.uz[gR] = mac.uz[bL];
+        }
     }
 }
 
@@ -553,34 +647,37 @@ __device__ __host__ __forceinline__ void face_extent(int axis, int nx, int ny, i
 
 template <int Q, class P, int KIND>
 __device__ __forceinline__ void face_op(const KParams &k, typename P::T *fout, float *gout, float *phi,
-                                        const Macro &mac, const uint64_t *counter, uint64_t step, int a, int b) {
+                                        const Macro &mac, const uint64_t *counter, uint64_t step, int a, int b,
+                                        bool macros) {
     if constexpr (KIND == kInflowY) inflow_face<Q, P, 1>(k, fout, gout, phi, mac, counter, step, a, b);
     if constexpr (KIND == kInflowZ) inflow_face<Q, P, 2>(k, fout, gout, phi, mac, counter, step, a, b);
     if constexpr (KIND == kOutflowY) outflow_face<Q, P, 1>(k, fout, gout, phi, mac, a, b);
     if constexpr (KIND == kOutflowZ) outflow_face<Q, P, 2>(k, fout, gout, phi, mac, a, b);
-    if constexpr (KIND == kPeriodicXRef) periodic_face<Q, P, 0, 1>(k, fout, gout, phi, mac, a, b);
-    if constexpr (KIND == kPeriodicXTrue) periodic_face<Q, P, 0, 2>(k, fout, gout, phi, mac, a, b);
-    if constexpr (KIND == kPeriodicYRef) periodic_face<Q, P, 1, 1>(k, fout, gout, phi, mac, a, b);
-    if constexpr (KIND == kPeriodicYTrue) periodic_face<Q, P, 1, 2>(k, fout, gout, phi, mac, a, b);
-    if constexpr (KIND == kPeriodicZRef) periodic_face<Q, P, 2, 1>(k, fout, gout, phi, mac, a, b);
-    if constexpr (KIND == kPeriodicZTrue) periodic_face<Q, P, 2, 2>(k, fout, gout, phi, mac, a, b);
+    if constexpr (KIND == kPeriodicXRef) periodic_face<0>(k, phi, mac, a, b, macros);
+    if constexpr (KIND == kPeriodicXTrue) periodic_face<0>(k, phi, mac, a, b, macros);
+    if constexpr (KIND == kPeriodicYRef) periodic_face<1>(k, phi, mac, a, b, macros);
+    if constexpr (KIND == kPeriodicYTrue) periodic_face<1>(k, phi, mac, a, b, macros);
+    if constexpr (KIND == kPeriodicZRef) periodic_face<2>(k, phi, mac, a, b, macros);
+    if constexpr (KIND == kPeriodicZTrue) periodic_face<2>(k, phi, mac, a, b, macros);
 }
 
+// flags: bit 0 = advance the device step counter (last face kernel of the
+// step), bit 1 = macro step (keep u current on periodic ghost layers).
 // One field set: 2D grid of 64x4 blocks over the face.  The inlet kinds read
 // the device step counter, so only the other kinds may advance it.
 template <int Q, class P, int KIND>
 __global__ void __launch_bounds__(256) k_face(KParams k, typename P::T *fout, float *gout, float *phi, Macro mac,
-                                              uint64_t *counter, uint64_t step, int bump_last) {
+                                              uint64_t *counter, uint64_t step, int flags) {
     const int a = blockIdx.x * 64 + threadIdx.x;
     const int b = blockIdx.y * 4 + threadIdx.y;
-    face_op<Q, P, KIND>(k, fout, gout, phi, mac, counter, step, a, b);
-    if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, bump_last);
+    face_op<Q, P, KIND>(k, fout, gout, phi, mac, counter, step, a, b, (flags & 2) != 0);
+    if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, flags & 1);
 }
 
 // Ensemble: 1D grid of 64x4 blocks over every jet's face blocks.
 template <int Q, class P, int KIND>
 __global__ void __launch_bounds__(256) k_face_batch(const JetSlot *__restrict__ jets, int njets, uint64_t *counter,
-                                                    uint64_t step, int bump_last) {
+                                                    uint64_t step, int flags) {
     constexpr int axis = face_axis(KIND);
     const int j = find_slot(jets, njets, blockIdx.x, 2 + axis);
     const JetSlot &js = jets[j];
@@ -591,8 +688,9 @@ __global__ void __launch_bounds__(256) k_face_batch(const JetSlot *__restrict__ 
     const int a = int(lb % uint32_t(bxs)) * 64 + threadIdx.x;
     const int b = int(lb / uint32_t(bxs)) * 4 + threadIdx.y;
     using PT = typename P::T;
-    face_op<Q, P, KIND>(js.k, static_cast<PT *>(js.f_next), js.g_next, js.phi, js.mac, counter, step, a, b);
-    if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, bump_last);
+    face_op<Q, P, KIND>(js.k, static_cast<PT *>(js.f_next), js.g_next, js.phi, js.mac, counter, step, a, b,
+                        (flags & 2) != 0);
+    if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, flags & 1);
 }
 
 #ifdef SSMD_STEP_KERNELS

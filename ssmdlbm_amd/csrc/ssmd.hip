@@ -63,6 +63,9 @@ KParams make_kparams(const ssmd_params &p) {
     };
     k.water = jet(p.water);
     k.oil = jet(p.oil);
+    k.wrap_x = p.wrap_x;
+    k.wrap_y = p.wrap_y;
+    k.wrap_z = p.wrap_z;
     return k;
 }
 
@@ -175,7 +178,8 @@ int launch_lattice(const ssmd_params &p, const ssmd_fields &d, uint32_t flags, h
     return launch_lattice_sel(p, d, flags, tile_of(p), p.lz, s);
 }
 
-int launch_boundaries(const ssmd_params &p, const ssmd_fields &d, uint64_t *counter, uint64_t step, hipStream_t s) {
+int launch_boundaries(const ssmd_params &p, const ssmd_fields &d, uint64_t *counter, uint64_t step, uint32_t sflags,
+                      hipStream_t s) {
     const KParams k = make_kparams(p);
     const Macro m = make_macro(d);
     int kinds[8];
@@ -191,8 +195,9 @@ int launch_boundaries(const ssmd_params &p, const ssmd_fields &d, uint64_t *coun
             face_extent(face_axis_host(kinds[i]), k.nx, k.ny, k.nz, na, nb);
             const dim3 grid((na + 63) / 64, (nb + 3) / 4);
             face_dispatch(kinds[i], [&]<int KIND>() {
+                const int fl = int(i == last) | ((sflags & SSMD_STEP_MACROS) ? 2 : 0);
                 hipLaunchKernelGGL((k_face<Q, P, KIND>), grid, dim3(64, 4), 0, s, k, fo, d.g_next, d.phi, m, counter,
-                                   step, int(i == last));
+                                   step, fl);
             });
         }
         if (counter && last < 0 && nk > 0) hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, counter);
@@ -205,7 +210,7 @@ int step_dispatch(const ssmd_params &p, const ssmd_fields &d, uint64_t *counter,
                   hipStream_t s) {
     SSMD_TRY(launch_phase(p, d, s));
     SSMD_TRY(launch_lattice(p, d, flags, s));
-    return launch_boundaries(p, d, counter, step, s);
+    return launch_boundaries(p, d, counter, step, flags, s);
 }
 
 void swap_buffers(ssmd_fields *d) {
@@ -299,6 +304,18 @@ int ssmd_validate(const ssmd_params *p) {
     if (p->wrap_z && (p->oil.enabled || p->outflow_z)) return SSMD_EINVAL;
     if (!(p->visc_water > 0.0) || !(p->visc_oil > 0.0)) return SSMD_EINVAL;
     if (p->tile < 0 || p->tile >= kNumTiles || p->lz < 0) return SSMD_EINVAL;
+    // Periodic wraps are folded into the pull addresses from the first step
+    // on, so the t=0 state must already be periodic along a wrapped axis:
+    // inlet discs (the only non-uniform part of it) keep clear of those faces.
+    auto disc_clear = [](const ssmd_jet &j, double lo_hi_n, bool axis_a) {
+        if (!j.enabled) return true;
+        const double r = std::sqrt(j.r2) + 1.0;
+        const double c = axis_a ? j.center_a : j.center_b;
+        return c - r > 2.0 && c + r < lo_hi_n - 3.0;
+    };
+    if (p->wrap_x && !(disc_clear(p->water, p->nx, true) && disc_clear(p->oil, p->nx, true))) return SSMD_EUNSUPPORTED;
+    if (p->wrap_y && !disc_clear(p->oil, p->ny, false)) return SSMD_EUNSUPPORTED;
+    if (p->wrap_z && !disc_clear(p->water, p->nz, false)) return SSMD_EUNSUPPORTED;
     return SSMD_OK;
 }
 
@@ -419,7 +436,7 @@ int ssmd_launch_boundaries(const ssmd_params *p, const ssmd_fields *d, uint64_t 
     SSMD_TRY(ssmd_validate(p));
     if (!fields_ready(d)) return SSMD_ENOTREADY;
     if (use_counter && !d->step_counter) return SSMD_ENOTREADY;
-    return launch_boundaries(*p, *d, use_counter ? d->step_counter : nullptr, step, static_cast<hipStream_t>(stream));
+    return launch_boundaries(*p, *d, use_counter ? d->step_counter : nullptr, step, 0, static_cast<hipStream_t>(stream));
 }
 
 int ssmd_swap(ssmd_fields *d) {
@@ -689,8 +706,9 @@ int ensemble_launch_step(ssmd_ensemble *e, const JetSlot *tab, int tile, uint32_
         for (int i = 0; i < e->nk; ++i) {
             const int ax = face_axis_host(e->kinds[i]);
             face_dispatch(e->kinds[i], [&]<int KIND>() {
+                const int fl = int(i == e->last) | (macros ? 2 : 0);
                 hipLaunchKernelGGL((k_face_batch<Q, P, KIND>), dim3(e->face_blocks[ax]), dim3(64, 4), 0, s, tab, e->n,
-                                   counter, step, int(i == e->last));
+                                   counter, step, fl);
             });
         }
         if (counter && e->last < 0 && e->nk > 0) hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, counter);

@@ -158,9 +158,9 @@ class Simulation:
         torch.cuda.synchronize(self.device)
 
     def pops_arrival(self, which: str = "f") -> np.ndarray:
-        """Same populations in the reference's arrival-indexed layout; entries
-        whose source cell lies outside the domain are NaN."""
-        return source_to_arrival(self.pops_source(which))
+        """Same populations in the reference's arrival-indexed layout (periodic
+        images folded in); entries whose source lies outside the domain are NaN."""
+        return source_to_arrival(self.pops_source(which), self.p)
 
     def phase_mass(self) -> np.ndarray:
         """FP64 [sum phi, sum rho ux, sum rho uy, sum rho uz] over interior cells."""
@@ -170,26 +170,42 @@ class Simulation:
         return np.array(out[:], dtype=np.float64)
 
 
-def source_to_arrival(src: np.ndarray) -> np.ndarray:
-    """arrival[i][a] = source[i][a - c_i] (NaN where a - c_i is outside)."""
+def _source_axis(n: int, c: int, wrap: int):
+    """Source coordinate of each arrival coordinate 0..n-1 along one axis for
+    a direction component c, with the periodic image folded in; -1 = none."""
+    s = np.arange(n) - c
+    valid = (s >= 0) & (s < n)
+    if wrap == abi.WRAP_REFERENCE:
+        s[(s == 0) & (c > 0)] = n - 3
+        s[(s == n - 1) & (c < 0)] = 2
+    elif wrap == abi.WRAP_TRUE:
+        s[(s == 0) & (c > 0)] = n - 2
+        s[(s == n - 1) & (c < 0)] = 1
+    s[~valid] = -1
+    return s
+
+
+def source_to_arrival(src: np.ndarray, params: abi.Params | None = None) -> np.ndarray:
+    """arrival[i][a] = source[i][a - c_i] (NaN where a - c_i is outside).
+
+    With `params`, a source slot on a periodic ghost face is replaced by its
+    periodic image, as the kernels read it (the wrap is part of the pull)."""
     q = src.shape[0]
     out = np.full(src.shape, np.nan)
     nz, ny, nx = src.shape[1:]
+    wx = int(params.wrap_x) if params is not None else 0
+    wy = int(params.wrap_y) if params is not None else 0
+    wz = int(params.wrap_z) if params is not None else 0
     for i in range(q):
         cx, cy, cz = DIRECTIONS[i]
-        # a = s + c  ->  a in [max(c,0), n + min(c,0))
-        az = slice(max(cz, 0), nz + min(cz, 0))
-        ay = slice(max(cy, 0), ny + min(cy, 0))
-        ax = slice(max(cx, 0), nx + min(cx, 0))
-        sz = slice(max(-cz, 0), nz + min(-cz, 0))
-        sy = slice(max(-cy, 0), ny + min(-cy, 0))
-        sx = slice(max(-cx, 0), nx + min(-cx, 0))
-        out[i][az, ay, ax] = src[i][sz, sy, sx]
+        sx, sy, sz = _source_axis(nx, cx, wx), _source_axis(ny, cy, wy), _source_axis(nz, cz, wz)
+        ax, ay, az = np.nonzero(sx >= 0)[0], np.nonzero(sy >= 0)[0], np.nonzero(sz >= 0)[0]
+        out[i][np.ix_(az, ay, ax)] = src[i][np.ix_(sz[az], sy[ay], sx[ax])]
     return out
 
 
 def arrival_to_source(arr: np.ndarray) -> np.ndarray:
-    """Inverse of source_to_arrival (0 where the arrival is outside)."""
+    """Inverse of source_to_arrival without wraps (0 where the arrival is outside)."""
     q = arr.shape[0]
     out = np.zeros(arr.shape)
     nz, ny, nx = arr.shape[1:]
