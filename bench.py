@@ -169,15 +169,40 @@ def cpu_baseline(case, budget_s: float = 15.0):
     }
 
 
-def profile_mode(case, steps: int) -> int:
-    """Eager steps (one dispatch per kernel, as counter collection wants them),
-    then torch copies of 2^28 floats = 1 GiB read + 1 GiB written."""
+def copy_bandwidth(reps: int = 10) -> float:
+    """Device-to-device copy of 2^28 floats (1 GiB read + 1 GiB written), GB/s:
+    the practical streaming ceiling the lattice kernel is compared with."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * src.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def profile_mode(case, steps: int, tune: bool) -> int:
+    """Tuned like the timed run, then `steps` eager steps (one dispatch per
+    kernel, as counter collection wants them), then torch copies of 2^28
+    floats = 1 GiB read + 1 GiB written (the byte-count reference)."""
     import torch
 
     from ssmdlbm_amd.solver import Simulation
 
     sim = Simulation(case.params(), "cuda:0")
     sim.init()
+    tile = (int(sim.p.tile), int(sim.p.lz))
+    if tune:
+        sim.step(2)
+        tile = sim.tune(reps=10)
     sim.step(steps)
     sim.synchronize()
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
@@ -185,7 +210,7 @@ def profile_mode(case, steps: int) -> int:
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    log(f"profile mode: {steps} steps of {case.name}, 3 copies of 1 GiB")
+    log(f"profile mode: tile {tile}, {steps} steps of {case.name} after tuning, 3 copies of 1 GiB")
     return 0
 
 
@@ -230,10 +255,12 @@ def main() -> int:
     abi.load()
     case = cases.WORKLOADS[args.workload]
     if args.profile_steps:
-        return profile_mode(case, args.profile_steps)
+        return profile_mode(case, args.profile_steps, tune=not args.no_tune)
     log(f"workload {case.name}: warmup {args.warmup}, timed {args.steps}")
     main_m = measure(case, args.steps, args.warmup, args.kernel_steps, tune=not args.no_tune)
     log(f"main: {main_m}")
+    copy_gbs = copy_bandwidth()
+    log(f"device copy: {copy_gbs:.0f} GB/s")
 
     forms = None
     if not args.no_forms:
@@ -290,6 +317,8 @@ def main() -> int:
             "kernel_ms": round(main_m["kernel_ms"], 5),
             "algorithmic_bytes_per_launch": main_m["interior_cells"] * kb,
             "step_frac_at_B_alg": round(main_m["step_frac"], 4),
+            "device_copy_gbs": round(copy_gbs, 1),
+            "frac_of_device_copy": round(main_m["kernel_gbs"] / copy_gbs, 4),
         },
         "cpu_baseline": cpu,
         "forms": forms,

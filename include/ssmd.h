@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define SSMD_ABI_VERSION 1
+#define SSMD_ABI_VERSION 2
 
 #define SSMD_OK 0
 #define SSMD_EINVAL (-1)       /* bad argument or inconsistent parameters */
@@ -60,6 +60,17 @@ enum { SSMD_POP_F32 = 0, SSMD_POP_F16 = 1, SSMD_POP_BF16 = 2 };
  *   TRUE      : populations re-enter with period n-2 (mass conserving);
  *               phi/u ghost wrap as REFERENCE. */
 enum { SSMD_WRAP_NONE = 0, SSMD_WRAP_REFERENCE = 1, SSMD_WRAP_TRUE = 2 };
+
+/* Subgrid closure of the hydrodynamic relaxation time (the reference has no
+ * subgrid model; all three are applied in the interior collision only -- the
+ * inlet/outflow reconstructions keep their fixed omco).  tau0 = 1/2 + 3 nu(phi).
+ *   NONE        : tau = tau0 (the reference, lbm.cu:229-231)
+ *   SMAGORINSKY : tau = (tau0 + sqrt(tau0^2 + 18 Cs^2 sqrt(2 Pi:Pi) / rho)) / 2,
+ *                 Pi the non-equilibrium moments of the step (Hou et al. 1996)
+ *   LEARNED     : tau = tau0 + tau_t[cell]; the step writes the network inputs
+ *                 of every interior cell to feat and then refreshes tau_t
+ *                 from them (ssmd_closure_update), so tau_t lags one step. */
+enum { SSMD_CLOSURE_NONE = 0, SSMD_CLOSURE_SMAGORINSKY = 1, SSMD_CLOSURE_LEARNED = 2 };
 
 /* ssmd_step flags */
 #define SSMD_STEP_MACROS 1u /* write rho, u, Pi and F at every interior cell */
@@ -96,6 +107,16 @@ typedef struct ssmd_params {
     uint64_t seed;       /* seed of that noise */
     ssmd_jet water;
     ssmd_jet oil;
+    int32_t closure;     /* SSMD_CLOSURE_* */
+    int32_t pad1;
+    double cs;           /* Smagorinsky constant (0.1) */
+    /* LEARNED: network inputs per cell, in order: the strain rate
+     * S_ab = -3 Pi_ab / (2 rho tau) times strain_scale (xx, yy, zz, xy, xz, yz),
+     * phi, and |grad phi-hat| (the masked gradient of the force) times
+     * grad_scale; output tau_t = tau_scale * max(0, network(inputs)). */
+    double strain_scale;
+    double grad_scale;
+    double tau_scale;
 } ssmd_params;
 
 /* Device buffers.  Member order follows struct LBMFields (LBMFields.cuh:26-85);
@@ -116,6 +137,10 @@ typedef struct ssmd_fields {
     uint64_t *step_counter; /* device word: inlet-noise step index for graph replays */
     double *reduce;   /* SSMD_REDUCE_DOUBLES doubles of reduction workspace */
     uint64_t owned;   /* nonzero when ssmd_alloc owns the buffers */
+    /* SSMD_CLOSURE_LEARNED only (ssmd_alloc allocates them for that closure): */
+    float *tau_t;     /* eddy relaxation time per cell, read by the collision */
+    uint16_t *feat;   /* 8 bf16 network inputs per cell (16-byte aligned rows) */
+    void *mlp;        /* packed network, ssmd_mlp_packed_bytes() (ssmd_mlp_pack) */
 } ssmd_fields;
 
 #define SSMD_REDUCE_DOUBLES 8192
@@ -131,7 +156,9 @@ const char *ssmd_error_string(int code);
  * oil jet on z=0, outflow on y and z, reference periodicX. */
 int ssmd_params_reference_default(ssmd_params *p, int32_t q);
 
-/* 0 if *p describes a case this build runs, else SSMD_EINVAL/SSMD_EUNSUPPORTED. */
+/* 0 if *p describes a case this build runs, else SSMD_EINVAL/SSMD_EUNSUPPORTED.
+ * Size limit: 4 * (nx*ny*nz + nx*ny) < 2^32 (the kernels address one
+ * population plane with 32-bit byte offsets), e.g. 1024 x 1024 x 1020. */
 int ssmd_validate(const ssmd_params *p);
 
 /* Bytes of one buffer: which = 0 scalar field, 1 f, 2 g. 0 on bad params. */
@@ -158,12 +185,16 @@ int ssmd_step(const ssmd_params *p, ssmd_fields *d, uint64_t step, uint32_t flag
  *   ssmd_launch_lattice     computeForces + computeMoments + streamCollide
  *                           (reads d->f, phi; writes d->f_next, d->g_next)
  *   ssmd_launch_boundaries  K5-K9 and the y/z periodic copies on d->f_next,
- *                           d->g_next (device step counter used if counter != 0)
+ *                           d->g_next (device step counter used if counter != 0;
+ *                           SSMD_STEP_MACROS in flags also refreshes rho/u on
+ *                           the ghost layer, as ssmd_step does on macro steps)
+ * A step run as these calls plus the LEARNED closure's ssmd_closure_update is
+ * the same step as ssmd_step with the same flags.
  *   ssmd_swap               host-side exchange f<->f_next, g<->g_next */
 int ssmd_launch_phase(const ssmd_params *p, const ssmd_fields *d, void *stream);
 int ssmd_launch_lattice(const ssmd_params *p, const ssmd_fields *d, uint32_t flags, void *stream);
 int ssmd_launch_boundaries(const ssmd_params *p, const ssmd_fields *d, uint64_t step, int32_t use_counter,
-                           void *stream);
+                           uint32_t flags, void *stream);
 int ssmd_swap(ssmd_fields *d);
 
 /* nsteps calls of ssmd_step for steps step0 .. step0+nsteps-1. */
@@ -228,6 +259,55 @@ int ssmd_write_bin(const ssmd_params *p, const float *device_field, const char *
 int ssmd_write_vti(const ssmd_params *p, const char *dir, uint64_t step, const char *const *names, int32_t count);
 int ssmd_write_info(const ssmd_params *p, const char *dir, const char *id, const char *velocity_set, uint64_t nsteps,
                     uint64_t save_every);
+
+/* ---------------------------------------------------------------------------
+ * Learned closure network (no counterpart in the reference).
+ * 8 inputs -> 64 -> 64 -> 64 (ReLU) -> 1, BF16 weights and activations on the
+ * matrix cores with FP32 accumulation.  Parameters are SSMD_MLP_PARAMS floats
+ * in this order: W1[64][8] (row = output unit), b1[64], W2[64][64], b2[64],
+ * W3[64][64], b3[64], W4[64], b4.  Inputs are bf16 bit patterns, 8 per sample,
+ * rows 16-byte aligned.  All pointers are device pointers; every call is
+ * asynchronous on `stream` and allocates nothing.
+ */
+#define SSMD_MLP_INPUTS 8
+#define SSMD_MLP_HIDDEN 64
+#define SSMD_MLP_PARAMS 8961
+
+/* Bytes of the packed (matrix-core operand order) form of a network. */
+size_t ssmd_mlp_packed_bytes(void);
+/* params (SSMD_MLP_PARAMS floats) -> packed. */
+int ssmd_mlp_pack(const float *params, void *packed, void *stream);
+/* y[c] = network(x[c]), c < n. */
+int ssmd_mlp_forward(const void *packed, const uint16_t *x, float *y, uint32_t n, void *stream);
+/* Workspace bytes ssmd_mlp_grad needs for n samples. */
+size_t ssmd_mlp_grad_work_bytes(uint32_t n);
+/* Gradient (SSMD_MLP_PARAMS floats) and value (one float) of the weighted
+ * mean squared error (1/n) sum_c w[c] (network(x[c]) - t[c])^2; w = NULL
+ * weighs every sample 1.  Deterministic (fixed reduction order). */
+int ssmd_mlp_grad(const void *packed, const uint16_t *x, const float *t, const float *w, uint32_t n, float *grad,
+                  float *loss, void *work, size_t work_bytes, void *stream);
+/* One Adam step (Kingma & Ba 2015, bias-corrected; step counts from 1) on
+ * params, with first/second moment buffers m, v (SSMD_MLP_PARAMS floats each). */
+int ssmd_mlp_adam(float *params, const float *grad, float *m, float *v, uint32_t step, float lr, float beta1,
+                  float beta2, float eps, void *stream);
+/* Training samples of the LEARNED closure from a fine field set whose last
+ * step wrote every moment (SSMD_STEP_MACROS), box-filtered by 2 onto the
+ * coarse grid `coarse` (acoustic scaling: coarse interior cell i covers fine
+ * cells 2i-1 and 2i per axis; requires 2 (n_c - 2) <= n_f - 2 on each axis).
+ * work receives 17 coarse planes (ssmd_sgs_work_bytes): rho, Favre-filtered
+ * ux uy uz, phi, the filtered strain rate S (xx yy zz xy xz yz, per coarse
+ * step) and the subgrid stress T = <rho u u> - <rho> u~ u~ (same order).
+ * Per coarse cell: feat = the 8 network inputs as the LEARNED step forms them
+ * (coarse->strain_scale, grad_scale), target = 3 nu_t / coarse->tau_scale with
+ * nu_t = -(dev T : S) / (2 rho S:S) the least-squares eddy viscosity, and
+ * weight = S:S (0 on the ghost layer), so that the weighted squared error of
+ * tau_t is the squared residual of the modelled subgrid stress. */
+size_t ssmd_sgs_work_bytes(const ssmd_params *coarse);
+int ssmd_sgs_samples(const ssmd_params *fine, const ssmd_fields *fd, const ssmd_params *coarse, float *work,
+                     uint16_t *feat, float *target, float *weight, void *stream);
+/* LEARNED closure: tau_t[c] = tau_scale * max(0, network(feat[c])) for every
+ * cell (ssmd_step and the step graphs call it after the boundaries). */
+int ssmd_closure_update(const ssmd_params *p, const ssmd_fields *d, void *stream);
 
 #ifdef __cplusplus
 }

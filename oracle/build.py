@@ -1,6 +1,7 @@
 """Build the CPU oracle (test infrastructure; see ssmd_oracle.cpp header).
 
-Produces oracle/_build/libssmd_oracle.so with g++ (-O3, OpenMP).  The
+Produces oracle/_build/libssmd_oracle.so with g++ (-O3, OpenMP) from
+ssmd_oracle.cpp (the lattice step) and mlp_oracle.cpp (the closure network).  The
 instruction set is pinned to x86-64-v3 so the library built here also runs
 on the host CPUs of the GPU machines.
 
@@ -24,7 +25,7 @@ def build_oracle(verbose: bool = True) -> Path:
     OUT.parent.mkdir(parents=True, exist_ok=True)
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [CXX, "-O3", "-march=x86-64-v3", "-fopenmp", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
-           "-I", str(ROOT / "include"), str(HERE / "ssmd_oracle.cpp"), "-o", str(tmp)]
+           "-I", str(ROOT / "include"), str(HERE / "ssmd_oracle.cpp"), str(HERE / "mlp_oracle.cpp"), "-o", str(tmp)]
     if verbose:
         print("[build]", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

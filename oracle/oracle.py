@@ -15,7 +15,8 @@ from ssmdlbm_amd import abi
 LIB_PATH = Path(__file__).resolve().parent / "_build" / "libssmd_oracle.so"
 
 FIELD_IDS = {n: i for i, n in enumerate(
-    ("rho", "ux", "uy", "uz", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz", "phi", "fsx", "fsy", "fsz"))}
+    ("rho", "ux", "uy", "uz", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz", "phi", "fsx", "fsy", "fsz", "tau_t")
+    + tuple(f"feat{j}" for j in range(8)))}
 
 _lib = None
 
@@ -60,6 +61,12 @@ def load() -> C.CDLL:
         "oracle_bf16_to_f32": (C.c_float, [C.c_uint16]),
         "oracle_splitmix64": (C.c_uint64, [C.c_uint64]),
         "oracle_noise": (C.c_float, [C.c_uint64] * 4),
+        "oracle_smag_tau": (C.c_double, [C.c_double, d, C.c_double, C.c_double]),
+        "oracle_mlp_params": (C.c_int, []),
+        "oracle_mlp_forward": (None, [d, d, C.c_int64, C.c_int, d]),
+        "oracle_mlp_grad": (None, [d, d, d, d, C.c_int64, C.c_int, d, d]),
+        "oracle_bf16_decode": (None, [C.POINTER(C.c_uint16), C.c_int64, d]),
+        "oracle_adam": (None, [d, d, d, d, C.c_int64, C.c_int64] + [C.c_double] * 4),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -148,3 +155,42 @@ class Oracle:
         out = np.zeros(4)
         self.lib.oracle_sums(self.h, _dptr(out))
         return out
+
+
+# -- closure network ----------------------------------------------------------
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def mlp_forward(params, x, quant: bool = True) -> np.ndarray:
+    """Network output per sample; x is (n, 8)."""
+    lib = load()
+    p, xx = _f64(params), _f64(x).reshape(-1, 8)
+    y = np.empty(xx.shape[0])
+    lib.oracle_mlp_forward(_dptr(p), _dptr(xx), xx.shape[0], int(quant), _dptr(y))
+    return y
+
+
+def mlp_grad(params, x, t, quant: bool = True, w=None):
+    """(gradient of the (weighted) mean squared error, loss)."""
+    lib = load()
+    p, xx, tt = _f64(params), _f64(x).reshape(-1, 8), _f64(t)
+    ww = None if w is None else _f64(w)
+    g = np.empty(lib.oracle_mlp_params())
+    loss = C.c_double()
+    lib.oracle_mlp_grad(_dptr(p), _dptr(xx), _dptr(tt), None if ww is None else _dptr(ww), xx.shape[0], int(quant),
+                        _dptr(g), C.byref(loss))
+    return g, loss.value
+
+
+def bf16_decode(bits) -> np.ndarray:
+    lib = load()
+    b = np.ascontiguousarray(bits, dtype=np.uint16)
+    out = np.empty(b.size)
+    lib.oracle_bf16_decode(b.ctypes.data_as(C.POINTER(C.c_uint16)), b.size, _dptr(out))
+    return out.reshape(b.shape)
+
+
+def adam(params, grad, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in place on float64 arrays."""
+    load().oracle_adam(_dptr(params), _dptr(_f64(grad)), _dptr(m), _dptr(v), params.size, step, lr, b1, b2, eps)

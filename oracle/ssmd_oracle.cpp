@@ -206,6 +206,15 @@ struct Lat {
     }
 };
 
+// Smagorinsky relaxation time (Hou et al. 1996): with nu_t = Cs^2 |S|,
+// |S| = sqrt(2 S:S) and S = -Pi / (2 rho cs^2 tau) (cs^2 = 1/3), the total
+// tau = tau0 + 3 nu_t solves tau^2 - tau0 tau - 4.5 Cs^2 sqrt(2 Pi:Pi) / rho = 0.
+template <class T>
+T smag_tau(T tau0, T pxx, T pyy, T pzz, T pxy, T pxz, T pyz, T rho, T cs) {
+    const T q = pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz);
+    return T(0.5) * (tau0 + std::sqrt(tau0 * tau0 + T(18) * cs * cs * std::sqrt(T(2) * q) / rho));
+}
+
 // ---------------------------------------------------------------------------
 struct Base {
     virtual ~Base() = default;
@@ -239,12 +248,13 @@ struct Oracle final : Base {
     uint32_t nx, ny, nz;
     size_t N;
     int threads = 0;
-    std::vector<T> fld[14];  // rho ux uy uz pxx pyy pzz pxy pxz pyz phi fsx fsy fsz
+    std::vector<T> fld[23];  // rho ux uy uz pxx pyy pzz pxy pxz pyz phi fsx fsy fsz,
+                             // tau_t, feat0..feat7 (closure; no reference counterpart)
     std::vector<T> f, g;
     std::vector<T> nnx, nny, nnz, nind;  // K2 normals (tile storage in the reference)
     T visc_water, visc_oil, visc_diff, omco_water, omco_oil, omco_ref, sigma, gamma;
 
-    enum { RHO, UX, UY, UZ, PXX, PYY, PZZ, PXY, PXZ, PYZ, PHI, FSX, FSY, FSZ };
+    enum { RHO, UX, UY, UZ, PXX, PYY, PZZ, PXY, PXZ, PYZ, PHI, FSX, FSY, FSZ, TAU_T, FEAT0, NFIELDS = FEAT0 + 8 };
 
     explicit Oracle(const ssmd_params &pp) : p(pp), L(pp.q), q(pp.q), nx(pp.nx), ny(pp.ny), nz(pp.nz) {
         N = size_t(nx) * ny * nz;
@@ -486,8 +496,14 @@ struct Oracle final : Base {
                     const T normx = gx * invInd, normy = gy * invInd, normz = gz * invInd;
                     const T ph = phi[c];
                     const T nu = std::fma(ph, visc_diff, visc_water);
-                    const T omega = T(1) / (T(0.5) + L.as2 * nu);
-                    const T omco = T(1) - omega;
+                    T omco;
+                    if (p.closure == SSMD_CLOSURE_NONE) {
+                        const T omega = T(1) / (T(0.5) + L.as2 * nu);
+                        omco = T(1) - omega;
+                    } else {
+                        omco = T(1) - T(1) / closure_tau(c, rho, pxx, pyy, pzz, pxy, pxz, pyz, ph,
+                                                         T(0.5) + L.as2 * nu);
+                    }
                     const T uu = T(1.5) * (ux * ux + uy * uy + uz * uz);
                     for (int i = 0; i < q; ++i) {
                         const T cx = T(C27[i][0]), cy = T(C27[i][1]), cz = T(C27[i][2]);
@@ -506,6 +522,20 @@ struct Oracle final : Base {
                         g[size_t(k) * N + dst] = geq + hi;
                     }
                 }
+    }
+
+    // Subgrid closures (no reference counterpart; the GPU kernel's
+    // closure_tau).  SMAGORINSKY: Hou et al. (1996), tau = (tau0 +
+    // sqrt(tau0^2 + 18 Cs^2 sqrt(2 Pi:Pi) / rho)) / 2.  LEARNED: tau = tau0 +
+    // tau_t; records the network inputs (strain rate S = -3 Pi / (2 rho tau)
+    // times strain_scale, phi, |grad phi-hat| times grad_scale), unrounded.
+    T closure_tau(size_t c, T rho, T pxx, T pyy, T pzz, T pxy, T pxz, T pyz, T ph, T tau0) {
+        if (p.closure == SSMD_CLOSURE_SMAGORINSKY) return smag_tau<T>(tau0, pxx, pyy, pzz, pxy, pxz, pyz, rho, T(p.cs));
+        const T tau = tau0 + fld[TAU_T][c];
+        const T sc = T(-1.5) * T(p.strain_scale) / (rho * tau);
+        const T v[8] = {pxx * sc, pyy * sc, pzz * sc, pxy * sc, pxz * sc, pyz * sc, ph, nind[c] * T(p.grad_scale)};
+        for (int j = 0; j < 8; ++j) fld[FEAT0 + j][c] = v[j];
+        return tau;
     }
 
     // -- boundaries (BoundaryConditions.cuh) -------------------------------
@@ -654,12 +684,12 @@ struct Oracle final : Base {
     // -- accessors ----------------------------------------------------------
     double *field(int) override { return nullptr; }
     int get_field(int id, double *out) override {
-        if (id < 0 || id >= 14) return SSMD_EINVAL;
+        if (id < 0 || id >= NFIELDS) return SSMD_EINVAL;
         for (size_t c = 0; c < N; ++c) out[c] = double(fld[id][c]);
         return 0;
     }
     int set_field(int id, const double *in) override {
-        if (id < 0 || id >= 14) return SSMD_EINVAL;
+        if (id < 0 || id >= NFIELDS) return SSMD_EINVAL;
         for (size_t c = 0; c < N; ++c) fld[id][c] = T(in[c]);
         return 0;
     }
@@ -753,6 +783,9 @@ double oracle_geq(int k, double phi, double ux, double uy, double uz) {
 double oracle_antidiff(int k, double sharp, double nx, double ny, double nz) {
     Lat<double> L(19);
     return L.antidiff(k, sharp, nx, ny, nz);
+}
+double oracle_smag_tau(double tau0, const double *pi6, double rho, double cs) {
+    return smag_tau<double>(tau0, pi6[0], pi6[1], pi6[2], pi6[3], pi6[4], pi6[5], rho, cs);
 }
 uint16_t oracle_f32_to_f16(float v) { return f32_to_f16(v); }
 float oracle_f16_to_f32(uint16_t h) { return f16_to_f32(h); }

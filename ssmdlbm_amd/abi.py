@@ -12,6 +12,9 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libssmd.so"
 
 POP_F32, POP_F16, POP_BF16 = 0, 1, 2
 WRAP_NONE, WRAP_REFERENCE, WRAP_TRUE = 0, 1, 2
+CLOSURE_NONE, CLOSURE_SMAGORINSKY, CLOSURE_LEARNED = 0, 1, 2
+MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 8, 64, 8961
+ABI_VERSION = 2
 STEP_MACROS = 1
 REDUCE_DOUBLES = 8192
 POP_BYTES = {POP_F32: 4, POP_F16: 2, POP_BF16: 2}
@@ -51,6 +54,12 @@ class Params(C.Structure):
         ("seed", C.c_uint64),
         ("water", Jet),
         ("oil", Jet),
+        ("closure", C.c_int32),
+        ("pad1", C.c_int32),
+        ("cs", C.c_double),
+        ("strain_scale", C.c_double),
+        ("grad_scale", C.c_double),
+        ("tau_scale", C.c_double),
     ]
 
     @property
@@ -81,6 +90,9 @@ class Fields(C.Structure):
         ("step_counter", C.c_void_p),
         ("reduce", C.c_void_p),
         ("owned", C.c_uint64),
+        ("tau_t", C.c_void_p),
+        ("feat", C.c_void_p),
+        ("mlp", C.c_void_p),
     ]
 
 
@@ -97,7 +109,8 @@ SIGNATURES = {
     "ssmd_step": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_uint64, C.c_uint32, C.c_void_p]),
     "ssmd_launch_phase": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_void_p]),
     "ssmd_launch_lattice": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_uint32, C.c_void_p]),
-    "ssmd_launch_boundaries": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_uint64, C.c_int32, C.c_void_p]),
+    "ssmd_launch_boundaries": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_uint64, C.c_int32, C.c_uint32,
+                                         C.c_void_p]),
     "ssmd_swap": (C.c_int, [C.POINTER(Fields)]),
     "ssmd_run": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_uint64, C.c_uint64, C.c_void_p]),
     "ssmd_graph_create": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -119,6 +132,18 @@ SIGNATURES = {
     "ssmd_write_bin": (C.c_int, [C.POINTER(Params), C.c_void_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_void_p]),
     "ssmd_write_vti": (C.c_int, [C.POINTER(Params), C.c_char_p, C.c_uint64, C.POINTER(C.c_char_p), C.c_int32]),
     "ssmd_write_info": (C.c_int, [C.POINTER(Params), C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64]),
+    "ssmd_mlp_packed_bytes": (C.c_size_t, []),
+    "ssmd_mlp_pack": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ssmd_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "ssmd_mlp_grad_work_bytes": (C.c_size_t, [C.c_uint32]),
+    "ssmd_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_size_t, C.c_void_p]),
+    "ssmd_mlp_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
+                                C.c_float, C.c_float, C.c_void_p]),
+    "ssmd_closure_update": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_void_p]),
+    "ssmd_sgs_work_bytes": (C.c_size_t, [C.POINTER(Params)]),
+    "ssmd_sgs_samples": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.POINTER(Params), C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 
@@ -136,12 +161,15 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise SsmdError(f"{LIB_PATH} is not built; run __graft_entry__.build() or python -m ssmdlbm_amd.build")
+    # torch loads its own HIP runtime; loading it first makes libssmd bind to
+    # that same runtime (a second copy sees no device and shares no memory).
+    import torch  # noqa: F401
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.ssmd_abi_version() != 1:
+    if lib.ssmd_abi_version() != ABI_VERSION:
         raise SsmdError("libssmd ABI version mismatch")
     _lib = lib
     return lib

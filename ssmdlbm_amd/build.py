@@ -15,7 +15,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIBDIR = PKG / "_lib"
 LIB = LIBDIR / "libssmd.so"
-SOURCES = [CSRC / "ssmd.hip", CSRC / "io.cpp"] + sorted(CSRC.glob("lattice_q*.hip"))
+SOURCES = [CSRC / "ssmd.hip", CSRC / "io.cpp", CSRC / "closure.hip"] + sorted(CSRC.glob("lattice_q*.hip"))
 DEPS = [CSRC / "lattice.hpp", CSRC / "kernels.hpp", CSRC / "launch.hpp", CSRC / "lattice_inst.hpp",
         ROOT / "include" / "ssmd.h"]
 OBJDIR = LIBDIR / "obj"

@@ -1,0 +1,443 @@
+"""Learned-closure study: training data from filtered fine-grid jets, training
+on the GPU, and the a-posteriori comparison of coarse runs.
+
+No counterpart in the reference (it has no subgrid model).  Geometry and
+scaling (acoustic: dx and dt halve on the fine grid, lattice velocities
+agree):
+
+  coarse  nx x ny x nz, tau_c, sigma_c, inlet diameter D, centre c
+  fine    2(n-2)+2.. cells per axis, tau_f = 1/2 + 2 (tau_c - 1/2),
+          sigma_f = 2 sigma_c, diameter 2D, centre 2c - 1/2
+
+so coarse interior cell i covers fine cells 2i-1, 2i (the box filter of
+ssmd_sgs_samples) and one coarse step is two fine steps.
+
+Statistics, on the coarse grid for both (fine: Favre-filtered fields):
+  centreline decay  time-averaged u_z on the four axis columns, per z;
+  interface spectrum  time- and slice-averaged radial power spectrum of the
+          interface-area density |grad phi| in horizontal slices.
+Errors are relative L2 norms of the coarse statistic against the filtered
+fine one.
+
+Command line (one GPU):
+  python -m ssmdlbm_amd.closure_study train --out DIR   fine runs -> network
+  python -m ssmdlbm_amd.closure_study eval  --net DIR/closure --out DIR
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import sys
+import time
+from dataclasses import asdict, dataclass
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import abi
+from .cases import gamma_from_tau_g, viscosity_from_tau
+from .closure import Network, Scales
+from .solver import Simulation
+
+
+@dataclass
+class Geometry:
+    nx: int = 128
+    ny: int = 128
+    nz: int = 512
+    q: int = 27
+    tau: float = 0.51
+    tau_g: float = 0.8
+    width: float = 4.0
+    diameter: int = 16
+    perturb: float = 0.01
+    cs: float = 0.1
+
+
+@dataclass
+class Jet:
+    u: float
+    sigma: float
+    seed: int
+
+
+def coarse_params(g: Geometry, jet: Jet, closure: int = abi.CLOSURE_NONE,
+                  scales: Scales | None = None) -> abi.Params:
+    p = abi.Params()
+    p.nx, p.ny, p.nz, p.q = g.nx, g.ny, g.nz, g.q
+    p.pop_format = abi.POP_F32
+    p.visc_water = p.visc_oil = viscosity_from_tau(g.tau)
+    p.sigma = jet.sigma
+    p.gamma = gamma_from_tau_g(g.tau_g, g.width)
+    p.perturb, p.seed = g.perturb, jet.seed
+    p.oil.enabled, p.oil.u, p.oil.phi = 1, jet.u, 1.0
+    p.oil.center_a, p.oil.center_b = (g.nx - 1) * 0.5, (g.ny - 1) * 0.5
+    r = g.diameter // 2
+    p.oil.r2 = float(r * r)
+    p.outflow_z = 1
+    p.wrap_x = p.wrap_y = abi.WRAP_REFERENCE
+    p.closure, p.cs = closure, g.cs
+    (scales or Scales()).apply(p)
+    return p
+
+
+def fine_params(g: Geometry, jet: Jet) -> abi.Params:
+    """The same jet at twice the resolution (acoustic scaling)."""
+    c = coarse_params(g, jet)
+    p = c.copy()
+    p.nx, p.ny, p.nz = 2 * g.nx, 2 * g.ny, 2 * g.nz
+    p.visc_water = p.visc_oil = viscosity_from_tau(0.5 + 2.0 * (g.tau - 0.5))
+    p.sigma = 2.0 * jet.sigma
+    p.oil.center_a, p.oil.center_b = 2.0 * c.oil.center_a - 0.5, 2.0 * c.oil.center_b - 0.5
+    r = g.diameter  # 2 * (D // 2) with D even
+    p.oil.r2 = float(r * r)
+    p.seed = jet.seed + 7919
+    p.closure = abi.CLOSURE_NONE
+    return p
+
+
+def log(msg: str) -> None:
+    print(f"[closure] {time.strftime('%H:%M:%S')} {msg}", flush=True)
+
+
+class Filter:
+    """ssmd_sgs_samples on device buffers for one (fine, coarse) pair."""
+
+    def __init__(self, fine: abi.Params, coarse: abi.Params, device="cuda:0"):
+        self.lib = abi.load()
+        self.fine, self.coarse = fine.copy(), coarse.copy()
+        nc = coarse.cells
+        self.nc = nc
+        self.work = torch.empty(int(self.lib.ssmd_sgs_work_bytes(C.byref(self.coarse))) // 4, device=device)
+        self.feat = torch.empty(nc * 8, dtype=torch.int16, device=device)
+        self.target = torch.empty(nc, device=device)
+        self.weight = torch.empty(nc, device=device)
+
+    def __call__(self, sim: Simulation) -> None:
+        abi.check(self.lib.ssmd_sgs_samples(C.byref(self.fine), C.byref(sim.d), C.byref(self.coarse),
+                                            self.work.data_ptr(), self.feat.data_ptr(), self.target.data_ptr(),
+                                            self.weight.data_ptr(), sim.stream_handle), "ssmd_sgs_samples")
+
+    def plane(self, k: int) -> torch.Tensor:
+        """Filtered coarse plane k as (nz, ny, nx) (0 rho, 1-3 u, 4 phi, 5-10 S, 11-16 T)."""
+        return self.work[k * self.nc:(k + 1) * self.nc].view(self.coarse.shape)
+
+
+class Stats:
+    """Running centreline and interface-spectrum statistics on a coarse grid."""
+
+    def __init__(self, p: abi.Params, z_range: tuple[int, int], kmax: int = 32, device="cuda:0"):
+        self.nx, self.ny = int(p.nx), int(p.ny)
+        cx, cy = (self.nx - 1) * 0.5, (self.ny - 1) * 0.5
+        self.ax = [int(np.floor(cx)), int(np.ceil(cx))]
+        self.ay = [int(np.floor(cy)), int(np.ceil(cy))]
+        self.z0, self.z1 = z_range
+        self.kmax = kmax
+        n = self.nx - 2
+        k = torch.fft.fftfreq(n, d=1.0 / n, device=device)
+        kr = torch.sqrt(k[None, :] ** 2 + k[:, None] ** 2).round().long()
+        self.bins = kr
+        self.centre = torch.zeros(int(p.nz), dtype=torch.float64, device=device)
+        self.spec = torch.zeros(kmax + 1, dtype=torch.float64, device=device)
+        self.area = torch.zeros(int(p.nz), dtype=torch.float64, device=device)
+        self.samples = 0
+
+    def add(self, uz: torch.Tensor, phi: torch.Tensor) -> None:
+        """uz, phi: (nz, ny, nx) coarse fields."""
+        col = uz[:, self.ay[0]:self.ay[1] + 1, self.ax[0]:self.ax[1] + 1].double().mean(dim=(1, 2))
+        self.centre += col
+        ph = phi.double()
+        inner = ph[:, 1:-1, 1:-1]
+        gx = torch.zeros_like(inner)
+        gy = torch.zeros_like(inner)
+        gz = torch.zeros_like(inner)
+        gx[:, :, 1:-1] = 0.5 * (inner[:, :, 2:] - inner[:, :, :-2])
+        gy[:, 1:-1, :] = 0.5 * (inner[:, 2:, :] - inner[:, :-2, :])
+        gz[1:-1] = 0.5 * (inner[2:] - inner[:-2])
+        a = torch.sqrt(gx ** 2 + gy ** 2 + gz ** 2)
+        self.area += a.sum(dim=(1, 2))
+        sl = a[self.z0:self.z1]
+        f = torch.fft.fft2(sl - sl.mean(dim=(1, 2), keepdim=True))
+        pw = (f.real ** 2 + f.imag ** 2).mean(dim=0)
+        b = self.bins.clamp(max=self.kmax + 1)
+        s = torch.zeros(self.kmax + 2, dtype=torch.float64, device=pw.device).scatter_add_(0, b.reshape(-1),
+                                                                                           pw.reshape(-1))
+        self.spec += s[:self.kmax + 1]
+        self.samples += 1
+
+    def result(self) -> dict:
+        n = max(self.samples, 1)
+        return {"centreline_uz": (self.centre / n).cpu().numpy().tolist(),
+                "spectrum": (self.spec / n).cpu().numpy().tolist(),
+                "area_per_plane": (self.area / n).cpu().numpy().tolist(), "samples": self.samples}
+
+
+def rel_l2(a, b) -> float:
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def compare(coarse: dict, filt: dict, z_range: tuple[int, int], kmax: int) -> dict:
+    z0, z1 = z_range
+    return {
+        "centreline_rel_l2": rel_l2(coarse["centreline_uz"][z0:z1], filt["centreline_uz"][z0:z1]),
+        "spectrum_rel_l2": rel_l2(coarse["spectrum"][1:kmax + 1], filt["spectrum"][1:kmax + 1]),
+        "interface_area_rel": float(abs(sum(coarse["area_per_plane"][z0:z1]) / sum(filt["area_per_plane"][z0:z1]) - 1)),
+    }
+
+
+# -- fine runs ----------------------------------------------------------------
+def run_fine(g: Geometry, jet: Jet, coarse_steps: int, sample_every: int, stat_from: int,
+             snapshot_steps: list[int], z_range, kmax, on_snapshot=None, tune: bool = True) -> dict:
+    """Fine run of `2 coarse_steps` steps; statistics of the filtered fields
+    every `sample_every` coarse steps from coarse step `stat_from`, and
+    on_snapshot(filter) at the listed coarse steps."""
+    fp, cp = fine_params(g, jet), coarse_params(g, jet)
+    sim = Simulation(fp)
+    sim.init()
+    if tune:
+        sim.step(2)
+        sim.tune(reps=4)
+    flt = Filter(fp, cp)
+    stats = Stats(cp, z_range, kmax)
+    marks = sorted(set([s for s in range(stat_from, coarse_steps + 1, sample_every)] + list(snapshot_steps)))
+    done = sim.step_index // 2
+    t0 = time.perf_counter()
+    for m in marks:
+        if 2 * m - 1 > sim.step_index:
+            sim.graph_run(2 * m - 1 - sim.step_index)
+        sim.step(1, macros=True)
+        flt(sim)
+        if m >= stat_from and (m - stat_from) % sample_every == 0:
+            stats.add(flt.plane(3), flt.plane(4))
+        if m in snapshot_steps and on_snapshot:
+            on_snapshot(flt)
+        if m - done >= 1000:
+            done = m
+            log(f"fine u={jet.u:.3f} sigma={jet.sigma:.3f}: coarse step {m}/{coarse_steps} "
+                f"({time.perf_counter() - t0:.0f} s)")
+    sim.synchronize()
+    if not torch.isfinite(sim.t["uz"]).all():
+        raise RuntimeError("fine run diverged")
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    return stats.result()
+
+
+def run_coarse(g: Geometry, jet: Jet, closure: int, net: Network | None, coarse_steps: int, sample_every: int,
+               stat_from: int, z_range, kmax, tune: bool = True) -> dict:
+    p = coarse_params(g, jet, closure, net.scales if net else None)
+    sim = Simulation(p)
+    if closure == abi.CLOSURE_LEARNED:
+        net.load_into(sim)
+    sim.init()
+    if tune:
+        sim.step(2)
+        sim.tune(reps=4)
+    stats = Stats(p, z_range, kmax)
+    t0 = time.perf_counter()
+    for m in range(stat_from, coarse_steps + 1, sample_every):
+        if m - 1 > sim.step_index:
+            sim.graph_run(m - 1 - sim.step_index)
+        sim.step(1, macros=True)
+        stats.add(sim.t["uz"].view(p.shape), sim.t["phi"].view(p.shape))
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    ok = bool(torch.isfinite(sim.t["uz"]).all())
+    res = stats.result()
+    res["finite"] = ok
+    if closure == abi.CLOSURE_LEARNED:
+        tt = sim.t["tau_t"]
+        res["tau_t_mean"] = float(tt.mean())
+        res["tau_t_max"] = float(tt.max())
+    res["seconds"] = wall
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    return res
+
+
+# -- training -----------------------------------------------------------------
+class SampleSet:
+    """Training samples collected on the device (cells with nonzero weight,
+    subsampled uniformly to at most `per_snapshot` per snapshot)."""
+
+    def __init__(self, per_snapshot: int, seed: int = 0):
+        self.per = per_snapshot
+        self.g = torch.Generator(device="cuda")
+        self.g.manual_seed(seed)
+        self.x, self.t, self.w = [], [], []
+
+    def add(self, flt: Filter) -> None:
+        keep = torch.nonzero(flt.weight > 0).squeeze(1)
+        if keep.numel() > self.per:
+            keep = keep[torch.randperm(keep.numel(), device=keep.device, generator=self.g)[:self.per]]
+        self.x.append(flt.feat.view(-1, 8)[keep].clone())
+        self.t.append(flt.target[keep].clone())
+        self.w.append(flt.weight[keep].clone())
+
+    def tensors(self):
+        return torch.cat(self.x), torch.cat(self.t), torch.cat(self.w)
+
+
+def fit_scales(g: Geometry, jets: list[Jet], probe_steps: int) -> Scales:
+    """Input/output scales from a short fine run of the first jet: 1/rms of
+    the filtered strain rate and of |grad phi|, rms of the target tau_t
+    (weighted)."""
+    jet = jets[0]
+    fp, cp = fine_params(g, jet), coarse_params(g, jet, abi.CLOSURE_LEARNED, Scales(1.0, 1.0, 1.0))
+    sim = Simulation(fp)
+    sim.init()
+    sim.graph_run(2 * probe_steps - 1)
+    sim.step(1, macros=True)
+    flt = Filter(fp, cp)
+    flt(sim)
+    w = flt.weight
+    m = w > 0
+    feats = flt.feat.view(-1, 8)[m].view(torch.bfloat16).float()
+    s_rms = float(torch.sqrt((feats[:, :6] ** 2).mean()))
+    g_rms = float(torch.sqrt((feats[:, 7] ** 2).mean()))
+    t = flt.target[m]
+    t_rms = float(torch.sqrt((w[m] * t ** 2).sum() / w[m].sum()))
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    return Scales(strain_scale=1.0 / max(s_rms, 1e-12), grad_scale=1.0 / max(g_rms, 1e-12),
+                  tau_scale=max(t_rms, 1e-6))
+
+
+def train_main(args) -> int:
+    out = Path(args.out)
+    out.mkdir(parents=True, exist_ok=True)
+    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q)
+    rng = np.random.default_rng(args.seed)
+    jets = [Jet(float(rng.uniform(0.03, 0.08)), float(rng.uniform(0.02, 0.1)), int(args.seed * 100 + j))
+            for j in range(args.runs)]
+    log(f"training jets: {[asdict(j) for j in jets]}")
+    scales = fit_scales(g, jets, max(args.steps // 4, 200))
+    log(f"scales: {asdict(scales)}")
+    samples = SampleSet(args.per_snapshot, seed=args.seed)
+    snaps = [int(s) for s in np.linspace(args.steps // 2, args.steps, args.snapshots)]
+    for jet in jets:
+        run_fine_scaled(g, jet, snaps, scales, samples.add)
+    x, t, w = samples.tensors()
+    w = w / w.mean()
+    n = x.shape[0]
+    log(f"{n} samples; target mean {float((w * t).sum() / w.sum()):.4f}")
+    # hold out a tenth for validation
+    perm = torch.randperm(n, device=x.device, generator=torch.Generator(device="cuda").manual_seed(1))
+    nv = n // 10
+    va, tr = perm[:nv], perm[nv:]
+    net = Network(seed=args.seed, scales=scales)
+    zero_loss = float((w[va] * t[va] ** 2).mean())
+    losses = net.train(x[tr], t[tr], steps=args.train_steps, batch=args.batch, lr=args.lr, lr_final=args.lr / 30,
+                       weight=w[tr], log_every=max(args.train_steps // 20, 1))
+    net.gradient(x[va], t[va], w[va])
+    val = float(net.loss.item())
+    # Smagorinsky's eddy relaxation time from the same inputs (strain columns)
+    s = x[va].view(torch.bfloat16).float()[:, :6] / scales.strain_scale
+    ss = (s[:, :3] ** 2).sum(1) + 2 * (s[:, 3:] ** 2).sum(1)
+    tau_smag = 3.0 * g.cs ** 2 * torch.sqrt(2 * ss) / scales.tau_scale
+    smag_loss = float((w[va] * (tau_smag - t[va]) ** 2).mean())
+    net.save(out / "closure")
+    rep = {"jets": [asdict(j) for j in jets], "scales": asdict(scales), "samples": n, "train_losses": losses,
+           "validation": {"learned": val, "no_closure": zero_loss, "smagorinsky": smag_loss}, "geometry": asdict(g),
+           "snapshots": snaps}
+    (out / "train_report.json").write_text(json.dumps(rep, indent=1))
+    log(f"validation weighted MSE: learned {val:.4g}, no closure {zero_loss:.4g}, Smagorinsky {smag_loss:.4g}")
+    return 0
+
+
+def run_fine_scaled(g: Geometry, jet: Jet, snaps: list[int], scales: Scales, on_snap) -> None:
+    """Fine run; at each listed coarse step the filter forms samples at the
+    training scales (its coarse parameters carry them) and on_snap gets it."""
+    fp = fine_params(g, jet)
+    cp = coarse_params(g, jet, abi.CLOSURE_LEARNED, scales)
+    sim = Simulation(fp)
+    sim.init()
+    sim.step(2)
+    sim.tune(reps=4)
+    flt = Filter(fp, cp)
+    t0 = time.perf_counter()
+    for m in snaps:
+        if 2 * m - 1 > sim.step_index:
+            sim.graph_run(2 * m - 1 - sim.step_index)
+        sim.step(1, macros=True)
+        flt(sim)
+        on_snap(flt)
+        log(f"fine u={jet.u:.3f} sigma={jet.sigma:.3f}: snapshot at coarse step {m} "
+            f"({time.perf_counter() - t0:.0f} s)")
+    sim.synchronize()
+    if not torch.isfinite(sim.t["uz"]).all():
+        raise RuntimeError("fine run diverged")
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+
+
+def eval_main(args) -> int:
+    out = Path(args.out)
+    out.mkdir(parents=True, exist_ok=True)
+    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q)
+    jet = Jet(args.u, args.sigma, args.seed)
+    net = Network.load(args.net)
+    z_range = (1, args.nz // 2)
+    kmax = args.kmax
+    stat_from = args.steps // 2
+    log(f"eval jet {asdict(jet)}; statistics every {args.every} coarse steps from {stat_from}")
+    res = {"jet": asdict(jet), "geometry": asdict(g), "steps": args.steps, "stat_from": stat_from,
+           "every": args.every, "z_range": z_range, "kmax": kmax}
+    for name, closure in (("none", abi.CLOSURE_NONE), ("smagorinsky", abi.CLOSURE_SMAGORINSKY),
+                          ("learned", abi.CLOSURE_LEARNED)):
+        r = run_coarse(g, jet, closure, net, args.steps, args.every, stat_from, z_range, kmax)
+        res[name] = r
+        log(f"coarse {name}: finite {r['finite']} ({r['seconds']:.1f} s)")
+    res["filtered_fine"] = run_fine(g, jet, args.steps, args.every, stat_from, [], z_range, kmax)
+    res["errors"] = {k: compare(res[k], res["filtered_fine"], z_range, kmax)
+                     for k in ("none", "smagorinsky", "learned")}
+    fine_cells = (2 * g.nx) * (2 * g.ny) * (2 * g.nz) * 2
+    res["cell_update_ratio"] = fine_cells / (g.nx * g.ny * g.nz)
+    (out / "eval_report.json").write_text(json.dumps(res))
+    for k, v in res["errors"].items():
+        log(f"{k:12s} centreline {v['centreline_rel_l2']:.4f}  spectrum {v['spectrum_rel_l2']:.4f}  "
+            f"interface area {v['interface_area_rel']:.4f}")
+    return 0
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    for name in ("train", "eval"):
+        s = sub.add_parser(name)
+        s.add_argument("--out", required=True)
+        s.add_argument("--nx", type=int, default=128)
+        s.add_argument("--ny", type=int, default=128)
+        s.add_argument("--nz", type=int, default=512)
+        s.add_argument("--q", type=int, default=27)
+        s.add_argument("--steps", type=int, default=10000, help="coarse steps (fine runs take twice as many)")
+        s.add_argument("--seed", type=int, default=5)
+    t = sub.choices["train"]
+    t.add_argument("--runs", type=int, default=2)
+    t.add_argument("--snapshots", type=int, default=6)
+    t.add_argument("--per-snapshot", type=int, default=1 << 20)
+    t.add_argument("--train-steps", type=int, default=3000)
+    t.add_argument("--batch", type=int, default=1 << 16)
+    t.add_argument("--lr", type=float, default=3e-3)
+    e = sub.choices["eval"]
+    e.add_argument("--net", required=True)
+    e.add_argument("--u", type=float, default=0.05)
+    e.add_argument("--sigma", type=float, default=0.05)
+    e.add_argument("--every", type=int, default=50)
+    e.add_argument("--kmax", type=int, default=32)
+    args = ap.parse_args(argv)
+    if not torch.cuda.is_available():
+        print("the closure study runs on the GPU", file=sys.stderr)
+        return 1
+    return train_main(args) if args.cmd == "train" else eval_main(args)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

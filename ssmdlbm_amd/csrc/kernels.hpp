@@ -38,10 +38,15 @@ struct KParams {
     uint64_t seed;
     JetK water, oil;
     int wrap_x, wrap_y, wrap_z;  // SSMD_WRAP_*
+    int closure;                 // SSMD_CLOSURE_*
+    float smag;                  // 18 Cs^2
+    float strain_scale, grad_scale;
 };
 
 struct Macro {
     float *rho, *ux, *uy, *uz, *pxx, *pyy, *pzz, *pxy, *pxz, *pyz, *fsx, *fsy, *fsz;
+    float *tau_t;    // LEARNED closure: eddy relaxation time (read)
+    uint16_t *feat;  // LEARNED closure: 8 bf16 network inputs per cell (written)
 };
 
 __device__ __forceinline__ uint32_t cell_id(const KParams &k, int x, int y, int z) {
@@ -102,14 +107,16 @@ __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restric
 // Fused lattice kernel.  A block of TX x TY threads owns an x-y tile and
 // marches over a chunk of z planes.  LDS rings hold 5 planes of phi-hat
 // (phi on interior cells, 0 on the ghost layer and outside; phaseField.cuh:
-// 125-142) with a 2-cell halo and 3 planes of the masked normals with a
+// 125-142) with a 2-cell halo and 4 planes of the masked normals with a
 // 1-cell halo: every thread gets the radius-2 neighbourhood of the force.
+// The fourth normal plane lets one barrier per z plane order the ring
+// writes of plane z+1 against the reads of plane z.
 template <int TX, int TY>
 struct LatticeSmem {
     static constexpr int HX = TX + 4, HY = TY + 4, HP = HX * HY;
     static constexpr int NXl = TX + 2, NYl = TY + 2, NP = NXl * NYl;
-    float m[5][HP];   // phi-hat ring
-    float4 n[3][NP];  // masked normal (x,y,z) and |grad phi-hat|
+    float m[5][HP];     // phi-hat ring
+    float n[4][4][NP];  // masked normal x, y, z and |grad phi-hat| (planes x component)
 };
 
 // Byte offset of the periodic image of a ghost source slot, for the
@@ -122,6 +129,36 @@ __device__ __forceinline__ int wrap_shift(int mode, int n) { return mode == SSMD
 // Correctly rounded sqrt/division are not needed here: one-ulp hardware ops.
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Relaxation time of a closure other than NONE (no reference counterpart).
+// tau0 = 1/2 + 3 nu(phi); `grad` = |grad phi-hat| of the cell.
+//   SMAGORINSKY: Hou et al. (1996), tau = (tau0 + sqrt(tau0^2 + 18 Cs^2
+//                sqrt(2 Pi:Pi) / rho)) / 2 with Pi the non-equilibrium moments.
+//   LEARNED:     tau = tau0 + tau_t; writes the network inputs of the cell:
+//                the strain rate S = -3 Pi / (2 rho tau) (scaled), phi and
+//                the scaled |grad phi-hat|, as 8 bf16 in one 16-byte store.
+__device__ __forceinline__ uint32_t bf16_pair(float a, float b) {
+    const __bf16 lo = static_cast<__bf16>(a), hi = static_cast<__bf16>(b);
+    return uint32_t(__builtin_bit_cast(uint16_t, lo)) | (uint32_t(__builtin_bit_cast(uint16_t, hi)) << 16);
+}
+__device__ __forceinline__ float closure_tau(const KParams &k, const Macro &mac, const Cell &m, uint32_t c, float ph,
+                                             float grad, float tau0) {
+    if (k.closure == SSMD_CLOSURE_SMAGORINSKY) {
+        const float q = m.pxx * m.pxx + m.pyy * m.pyy + m.pzz * m.pzz +
+                        2.0f * (m.pxy * m.pxy + m.pxz * m.pxz + m.pyz * m.pyz);
+        return 0.5f * (tau0 + __builtin_amdgcn_sqrtf(tau0 * tau0 + k.smag * __builtin_amdgcn_sqrtf(2.0f * q) *
+                                                                         __builtin_amdgcn_rcpf(m.rho)));
+    }
+    const float tau = tau0 + mac.tau_t[c];
+    const float s = -1.5f * k.strain_scale * __builtin_amdgcn_rcpf(m.rho * tau);
+    uint4 w;
+    w.x = bf16_pair(m.pxx * s, m.pyy * s);
+    w.y = bf16_pair(m.pzz * s, m.pxy * s);
+    w.z = bf16_pair(m.pxz * s, m.pyz * s);
+    w.w = bf16_pair(ph, grad * k.grad_scale);
+    *reinterpret_cast<uint4 *>(mac.feat + size_t(c) * 8) = w;
+    return tau;
+}
 
 // One tile: cells x0..x0+TX-1, y0..y0+TY-1, planes zb..ze-1 (threadIdx = (tx, ty)).
 // TX is a multiple of 64, so a wavefront covers part of one x row: y is
@@ -184,7 +221,7 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         }
     };
     auto normals = [&](int p) {
-        const int sl = (p + 3) % 3;
+        const int sl = p % 4;
         const int pm = (p + 4) % 5, p0 = (p + 5) % 5, pp = (p + 6) % 5;
         for (int e = tid; e < NP; e += NT) {
             const int hx = e % NXl + 1, hy = e / NXl + 1;
@@ -196,13 +233,16 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             iso_grad<Q>(at, gx, gy, gz);
             const float ind = fast_sqrt(gx * gx + gy * gy + gz * gz);
             const float inv = fast_rcp(ind + 1e-9f);
-            s_n[sl][e] = make_float4(gx * inv, gy * inv, gz * inv, ind);
+            s_n[sl][0][e] = gx * inv;
+            s_n[sl][1][e] = gy * inv;
+            s_n[sl][2][e] = gz * inv;
+            s_n[sl][3][e] = ind;
         }
     };
 
     {
         float r[EPT];
-        for (int p = zb - 2; p <= zb + 1; ++p) {
+        for (int p = zb - 2; p <= zb + 2; ++p) {
             fetch_plane(p, r);
             store_plane(p, r);
         }
@@ -211,66 +251,89 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
     normals(zb - 1);
     normals(zb);
     float nxt[EPT];
-    fetch_plane(zb + 2, nxt);  // one plane ahead, in registers
+    if (zb + 1 < ze) fetch_plane(zb + 3, nxt);  // one plane ahead, in registers
+    __syncthreads();  // plane zb-2 (read above) is the first ring slot reused
 
-    for (int z = zb; z < ze; ++z) {
-        // (1) the prefetched phi plane z+2 into LDS; start fetching z+3
-        store_plane(z + 2, nxt);
-        if (z + 1 < ze) fetch_plane(z + 3, nxt);
-
-        // (2) pull this plane's populations.  The plane stride is made opaque
-        //     per iteration so the 2Q uniform plane offsets are recomputed on
-        //     the scalar unit instead of being held (and spilled) across the loop.
+    // Pull the Q populations arriving at (x, y, zz).  The plane stride is
+    // made opaque per call so the 2Q uniform plane offsets are recomputed on
+    // the scalar unit instead of being held (and spilled) across the loop.
+    auto pull = [&](int zz, float (&f)[Q]) {
         size_t N = N0;
         asm volatile("" : "+s"(N));
-        const uint32_t c = cell_id(k, x, y, z);
+        const uint32_t c = cell_id(k, x, y, zz);
         long zsp = 0, zsm = 0;
         if (k.wrap_z) {
             const long sh = long(wrap_shift(k.wrap_z, k.nz)) * long(sxy);
-            if (z == 1) zsp = sh;
-            if (z == k.nz - 2) zsm = -sh;
+            if (zz == 1) zsp = sh;
+            if (zz == k.nz - 2) zsm = -sh;
         }
-        float f[Q];
-        if (active) {
-            const PT *p0 = fin + c;
-            const PT *pp = p0 + xsp;  // sources of c_x > 0 directions
-            const PT *pm = p0 + xsm;  // sources of c_x < 0 directions
-            sfor<0, Q>([&](auto I) {
-                constexpr int i = I;
-                constexpr Dir d = kDir[i];
-                long shift = long(i) * long(N) - long(d.x) - long(d.y) * k.nx - long(d.z) * long(sxy);
-                if constexpr (d.y > 0) shift += ysp;
-                if constexpr (d.y < 0) shift += ysm;
-                if constexpr (d.z > 0) shift += zsp;
-                if constexpr (d.z < 0) shift += zsm;
-                const PT *b = d.x > 0 ? pp : (d.x < 0 ? pm : p0);
-                f[i] = P::load(b[shift]);
-            });
-        }
-        __syncthreads();
+        // uniform base per direction + one of three shared 32-bit byte offsets
+        const uint32_t o0 = c * uint32_t(sizeof(PT));
+        const uint32_t op = uint32_t(int(c) + xsp) * uint32_t(sizeof(PT));  // c_x > 0 sources
+        const uint32_t om = uint32_t(int(c) + xsm) * uint32_t(sizeof(PT));  // c_x < 0 sources
+        sfor<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr Dir d = kDir[i];
+            long shift = long(i) * long(N) - long(d.x) - long(d.y) * k.nx - long(d.z) * long(sxy);
+            if constexpr (d.y > 0) shift += ysp;
+            if constexpr (d.y < 0) shift += ysm;
+            if constexpr (d.z > 0) shift += zsp;
+            if constexpr (d.z < 0) shift += zsm;
+            const char *base = reinterpret_cast<const char *>(fin + shift);
+            const uint32_t off = d.x > 0 ? op : (d.x < 0 ? om : o0);
+            f[i] = P::load(*reinterpret_cast<const PT *>(base + off));
+        });
+    };
+    // The populations of plane z+1 are in flight while plane z collides:
+    // the regularized collision needs only the moments, so f is dead once
+    // they are formed and its registers take the next plane.
+    float f[Q];
+    if (active) pull(zb, f);
+
+    for (int z = zb; z < ze; ++z) {
+        // (1) normals of plane z+1 (phi-hat planes z..z+2) into ring slot
+        //     (z+1)%4; then the prefetched phi plane z+3 into the slot of
+        //     plane z-2 and the fetch of z+4.  Before the barrier a wave may
+        //     still be in (2) of plane z-1, which reads normal slots z-2..z
+        //     and phi slot z-1: none of these is written here.
         normals(z + 1);
+        if (z + 1 < ze) {
+            store_plane(z + 3, nxt);
+            if (z + 2 < ze) fetch_plane(z + 4, nxt);
+        }
+        const uint32_t c = cell_id(k, x, y, z);
         __syncthreads();
         if (!active) continue;
 
         // (3) surface-tension force from the masked normals (phaseField.cuh:246-293)
-        const int nzm = (z + 2) % 3, nz0 = z % 3, nzp = (z + 1) % 3;
+        // (2) everything below reads normal slots z-1..z+1 and phi slot z
+        const int nzm = (z + 3) % 4, nz0 = z % 4, nzp = (z + 1) % 4;
         const int ncen = (ty + 1) * NXl + tx + 1;
         float kx, ky, kz;
-        const float4 nc = s_n[nz0][ncen];
+        const float4 nc = make_float4(s_n[nz0][0][ncen], s_n[nz0][1][ncen], s_n[nz0][2][ncen], s_n[nz0][3][ncen]);
         {
-            auto nat = [&](int dx, int dy, int dz) {
+            // component a of the masked normal at offset (dx, dy, dz)
+            auto nat = [&](int a, int dx, int dy, int dz) {
                 const int s = dz < 0 ? nzm : (dz > 0 ? nzp : nz0);
-                return s_n[s][ncen + dy * NXl + dx];
+                return s_n[s][a][ncen + dy * NXl + dx];
             };
             float acc[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
             sfor<0, (Q - 1) / 2>([&](auto Pp) {
                 constexpr int i = 2 * Pp + 1;
                 constexpr Dir d = kDir[i];
                 constexpr int cls = i <= 6 ? 0 : (i <= 18 ? 1 : 2);
-                const float4 a = nat(d.x, d.y, d.z), b = nat(-d.x, -d.y, -d.z);
-                if constexpr (d.x != 0) acc[cls][0] += d.x > 0 ? a.x - b.x : b.x - a.x;
-                if constexpr (d.y != 0) acc[cls][1] += d.y > 0 ? a.y - b.y : b.y - a.y;
-                if constexpr (d.z != 0) acc[cls][2] += d.z > 0 ? a.z - b.z : b.z - a.z;
+                if constexpr (d.x != 0) {
+                    const float a = nat(0, d.x, d.y, d.z), b = nat(0, -d.x, -d.y, -d.z);
+                    acc[cls][0] += d.x > 0 ? a - b : b - a;
+                }
+                if constexpr (d.y != 0) {
+                    const float a = nat(1, d.x, d.y, d.z), b = nat(1, -d.x, -d.y, -d.z);
+                    acc[cls][1] += d.y > 0 ? a - b : b - a;
+                }
+                if constexpr (d.z != 0) {
+                    const float a = nat(2, d.x, d.y, d.z), b = nat(2, -d.x, -d.y, -d.z);
+                    acc[cls][2] += d.z > 0 ? a - b : b - a;
+                }
             });
             constexpr float w1 = weight<Q>(1), w2 = weight<Q>(7), w3 = Q == 27 ? weight<Q>(19) : 0.f;
             float sx = w1 * acc[0][0] + w2 * acc[1][0];
@@ -337,15 +400,25 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             nz = gz * inv;
         }
 
-        // (5) moments, regularized collision, phase populations
+        // (5) moments; then the next plane's pull; then the collision
         const Cell m = moments<Q>(f, kx, ky, kz);
+        if (z + 1 < ze) pull(z + 1, f);
         const float nu = __builtin_fmaf(ph, k.visc_diff, k.visc_water);
-        const float omco = 1.0f - fast_rcp(0.5f + 3.0f * nu);
-        PT *fo = fout + c;
-        collide<Q>(m, omco, kx, ky, kz, [&](int i, float v) { fo[size_t(i) * N] = P::store(v); });
+        const float omco = 1.0f - fast_rcp(k.closure == SSMD_CLOSURE_NONE
+                                               ? 0.5f + 3.0f * nu
+                                               : closure_tau(k, mac, m, c, ph, nc.w, 0.5f + 3.0f * nu));
+        // uniform plane base + shared 32-bit byte offset for every store
+        size_t Ns = N0;
+        asm volatile("" : "+s"(Ns));
+        const uint32_t so = c * uint32_t(sizeof(PT));
+        collide<Q>(m, omco, kx, ky, kz, [&](int i, float v) {
+            *reinterpret_cast<PT *>(reinterpret_cast<char *>(fout + size_t(i) * Ns) + so) = P::store(v);
+        });
         const float sharp = k.gamma * ph * (1.0f - ph);
-        float *go = gout + c;
-        phase_collide(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) { go[size_t(kk) * N] = v; });
+        const uint32_t go = c * 4u;
+        phase_collide(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) {
+            *reinterpret_cast<float *>(reinterpret_cast<char *>(gout + size_t(kk) * Ns) + go) = v;
+        });
 
         const bool face = fx_lo || fx_hi || fy_lo || fy_hi || fz_lo || fz_hi;
         if (MACROS || face) {

@@ -66,6 +66,10 @@ KParams make_kparams(const ssmd_params &p) {
     k.wrap_x = p.wrap_x;
     k.wrap_y = p.wrap_y;
     k.wrap_z = p.wrap_z;
+    k.closure = p.closure;
+    k.smag = float(18.0 * p.cs * p.cs);
+    k.strain_scale = float(p.strain_scale);
+    k.grad_scale = float(p.grad_scale);
     return k;
 }
 
@@ -84,12 +88,18 @@ Macro make_macro(const ssmd_fields &d) {
     m.fsx = d.fsx;
     m.fsy = d.fsy;
     m.fsz = d.fsz;
+    m.tau_t = d.tau_t;
+    m.feat = d.feat;
     return m;
 }
 
-bool fields_ready(const ssmd_fields *d) {
-    return d && d->rho && d->ux && d->uy && d->uz && d->pxx && d->pyy && d->pzz && d->pxy && d->pxz && d->pyz &&
-           d->phi && d->fsx && d->fsy && d->fsz && d->f && d->g && d->f_next && d->g_next;
+bool fields_ready(const ssmd_params &p, const ssmd_fields *d) {
+    if (!(d && d->rho && d->ux && d->uy && d->uz && d->pxx && d->pyy && d->pzz && d->pxy && d->pxz && d->pyz &&
+          d->phi && d->fsx && d->fsy && d->fsz && d->f && d->g && d->f_next && d->g_next))
+        return false;
+    if (p.closure == SSMD_CLOSURE_LEARNED)
+        return d->tau_t && d->feat && d->mlp && (reinterpret_cast<uintptr_t>(d->feat) & 15) == 0;
+    return true;
 }
 
 int tile_of(const ssmd_params &p) { return (p.tile >= 0 && p.tile < kNumTiles) ? p.tile : 0; }
@@ -210,7 +220,10 @@ int step_dispatch(const ssmd_params &p, const ssmd_fields &d, uint64_t *counter,
                   hipStream_t s) {
     SSMD_TRY(launch_phase(p, d, s));
     SSMD_TRY(launch_lattice(p, d, flags, s));
-    return launch_boundaries(p, d, counter, step, flags, s);
+    SSMD_TRY(launch_boundaries(p, d, counter, step, flags, s));
+    if (p.closure == SSMD_CLOSURE_LEARNED)
+        return launch_closure_update(d.mlp, d.feat, d.tau_t, p.nx * p.ny * p.nz, float(p.tau_scale), s);
+    return SSMD_OK;
 }
 
 void swap_buffers(ssmd_fields *d) {
@@ -296,8 +309,10 @@ int ssmd_validate(const ssmd_params *p) {
     if (p->q != 19 && p->q != 27) return SSMD_EINVAL;
     if (p->pop_format < 0 || p->pop_format > 2) return SSMD_EINVAL;
     if (p->nx < 5 || p->ny < 5 || p->nz < 5) return SSMD_EINVAL;
+    // The kernels address a population or phase plane with 32-bit byte
+    // offsets (4-byte elements at most), one plane of slack for the pulls.
     const uint64_t n = uint64_t(p->nx) * p->ny * p->nz;
-    if (n >= (uint64_t(1) << 32) - uint64_t(p->nx) * p->ny) return SSMD_EUNSUPPORTED;
+    if (4 * (n + uint64_t(p->nx) * p->ny) >= (uint64_t(1) << 32)) return SSMD_EUNSUPPORTED;
     for (int w : {p->wrap_x, p->wrap_y, p->wrap_z})
         if (w < SSMD_WRAP_NONE || w > SSMD_WRAP_TRUE) return SSMD_EINVAL;
     if (p->wrap_y && (p->water.enabled || p->outflow_y)) return SSMD_EINVAL;
@@ -316,6 +331,11 @@ int ssmd_validate(const ssmd_params *p) {
     if (p->wrap_x && !(disc_clear(p->water, p->nx, true) && disc_clear(p->oil, p->nx, true))) return SSMD_EUNSUPPORTED;
     if (p->wrap_y && !disc_clear(p->oil, p->ny, false)) return SSMD_EUNSUPPORTED;
     if (p->wrap_z && !disc_clear(p->water, p->nz, false)) return SSMD_EUNSUPPORTED;
+    if (p->closure < SSMD_CLOSURE_NONE || p->closure > SSMD_CLOSURE_LEARNED) return SSMD_EINVAL;
+    if (p->closure == SSMD_CLOSURE_SMAGORINSKY && !(p->cs >= 0.0)) return SSMD_EINVAL;
+    if (p->closure == SSMD_CLOSURE_LEARNED &&
+        !(p->tau_scale >= 0.0 && std::isfinite(p->strain_scale) && std::isfinite(p->grad_scale)))
+        return SSMD_EINVAL;
     return SSMD_OK;
 }
 
@@ -366,6 +386,18 @@ int ssmd_alloc(const ssmd_params *p, ssmd_fields *d) {
     if (e != hipSuccess) return fail(e);
     e = hipMalloc(reinterpret_cast<void **>(&d->reduce), SSMD_REDUCE_DOUBLES * sizeof(double));
     if (e != hipSuccess) return fail(e);
+    if (p->closure == SSMD_CLOSURE_LEARNED) {
+        // zero weights: the network outputs 0 (tau_t = 0) until ssmd_mlp_pack
+        const size_t n = size_t(p->nx) * p->ny * p->nz;
+        void **bufs[] = {reinterpret_cast<void **>(&d->tau_t), reinterpret_cast<void **>(&d->feat), &d->mlp};
+        const size_t bytes[] = {n * sizeof(float), n * SSMD_MLP_INPUTS * sizeof(uint16_t), ssmd_mlp_packed_bytes()};
+        for (int i = 0; i < 3; ++i) {
+            e = hipMalloc(bufs[i], bytes[i]);
+            if (e != hipSuccess) return fail(e);
+            e = hipMemset(*bufs[i], 0, bytes[i]);
+            if (e != hipSuccess) return fail(e);
+        }
+    }
     return SSMD_OK;
 }
 
@@ -374,7 +406,7 @@ int ssmd_free(ssmd_fields *d) {
     if (!d->owned) return SSMD_OK;
     void *ptrs[] = {d->rho, d->ux,  d->uy,  d->uz,  d->pxx, d->pyy, d->pzz,    d->pxy,         d->pxz,
                     d->pyz, d->phi, d->fsx, d->fsy, d->fsz, d->f,   d->f_next, d->g,           d->g_next,
-                    d->step_counter, d->reduce};
+                    d->step_counter, d->reduce, d->tau_t, d->feat, d->mlp};
     int rc = 0;
     for (void *q : ptrs)
         if (q) {
@@ -387,7 +419,7 @@ int ssmd_free(ssmd_fields *d) {
 
 int ssmd_init(const ssmd_params *p, ssmd_fields *d, void *stream) {
     SSMD_TRY(ssmd_validate(p));
-    if (!fields_ready(d)) return SSMD_ENOTREADY;
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
     const KParams k = make_kparams(*p);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const dim3 grid((k.N + 255) / 256), block(256);
@@ -413,30 +445,39 @@ int ssmd_init(const ssmd_params *p, ssmd_fields *d, void *stream) {
 
 int ssmd_step(const ssmd_params *p, ssmd_fields *d, uint64_t step, uint32_t flags, void *stream) {
     SSMD_TRY(ssmd_validate(p));
-    if (!fields_ready(d)) return SSMD_ENOTREADY;
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
     SSMD_TRY(step_dispatch(*p, *d, nullptr, step, flags, static_cast<hipStream_t>(stream)));
     swap_buffers(d);
     return SSMD_OK;
 }
 
+int ssmd_closure_update(const ssmd_params *p, const ssmd_fields *d, void *stream) {
+    SSMD_TRY(ssmd_validate(p));
+    if (p->closure != SSMD_CLOSURE_LEARNED) return SSMD_EINVAL;
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
+    return launch_closure_update(d->mlp, d->feat, d->tau_t, p->nx * p->ny * p->nz, float(p->tau_scale),
+                                 static_cast<hipStream_t>(stream));
+}
+
 int ssmd_launch_phase(const ssmd_params *p, const ssmd_fields *d, void *stream) {
     SSMD_TRY(ssmd_validate(p));
-    if (!fields_ready(d)) return SSMD_ENOTREADY;
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
     return launch_phase(*p, *d, static_cast<hipStream_t>(stream));
 }
 
 int ssmd_launch_lattice(const ssmd_params *p, const ssmd_fields *d, uint32_t flags, void *stream) {
     SSMD_TRY(ssmd_validate(p));
-    if (!fields_ready(d)) return SSMD_ENOTREADY;
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
     return launch_lattice(*p, *d, flags, static_cast<hipStream_t>(stream));
 }
 
 int ssmd_launch_boundaries(const ssmd_params *p, const ssmd_fields *d, uint64_t step, int32_t use_counter,
-                           void *stream) {
+                           uint32_t flags, void *stream) {
     SSMD_TRY(ssmd_validate(p));
-    if (!fields_ready(d)) return SSMD_ENOTREADY;
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
     if (use_counter && !d->step_counter) return SSMD_ENOTREADY;
-    return launch_boundaries(*p, *d, use_counter ? d->step_counter : nullptr, step, 0, static_cast<hipStream_t>(stream));
+    return launch_boundaries(*p, *d, use_counter ? d->step_counter : nullptr, step, flags,
+                             static_cast<hipStream_t>(stream));
 }
 
 int ssmd_swap(ssmd_fields *d) {
@@ -452,7 +493,7 @@ int ssmd_run(const ssmd_params *p, ssmd_fields *d, uint64_t step0, uint64_t nste
 
 int ssmd_graph_create(const ssmd_params *p, const ssmd_fields *d, void *stream, ssmd_graph **out) {
     SSMD_TRY(ssmd_validate(p));
-    if (!fields_ready(d) || !out || !d->step_counter) return SSMD_ENOTREADY;
+    if (!fields_ready(*p, d) || !out || !d->step_counter) return SSMD_ENOTREADY;
     hipStream_t s = static_cast<hipStream_t>(stream);
     ssmd_graph *g = new (std::nothrow) ssmd_graph();
     if (!g) return SSMD_EINVAL;
@@ -518,7 +559,7 @@ int ssmd_graph_destroy(ssmd_graph *g) {
 
 int ssmd_phase_mass_f64(const ssmd_params *p, const ssmd_fields *d, double *out_host, void *stream) {
     SSMD_TRY(ssmd_validate(p));
-    if (!fields_ready(d) || !d->reduce) return SSMD_ENOTREADY;
+    if (!fields_ready(*p, d) || !d->reduce) return SSMD_ENOTREADY;
     if (!out_host) return SSMD_EINVAL;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const KParams k = make_kparams(*p);
@@ -591,7 +632,7 @@ extern "C" {
 int ssmd_tune(const ssmd_params *p, const ssmd_fields *d, int32_t reps, void *stream, int32_t *best_tile,
               int32_t *best_lz) {
     SSMD_TRY(ssmd_validate(p));
-    if (!fields_ready(d)) return SSMD_ENOTREADY;
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
     if (!best_tile || !best_lz || reps < 1) return SSMD_EINVAL;
     hipStream_t s = static_cast<hipStream_t>(stream);
     SSMD_TRY(launch_phase(*p, *d, s));
@@ -731,7 +772,9 @@ int ssmd_ensemble_create(const ssmd_params *params, const ssmd_fields *fields, i
     const int nk0 = face_kinds(params[0], kinds0);
     for (int j = 0; j < n; ++j) {
         SSMD_TRY(ssmd_validate(&params[j]));
-        if (!fields_ready(&fields[j])) return SSMD_ENOTREADY;
+        if (!fields_ready(params[j], &fields[j])) return SSMD_ENOTREADY;
+        // the batched step has no per-jet network pass
+        if (params[j].closure == SSMD_CLOSURE_LEARNED) return SSMD_EUNSUPPORTED;
         if (params[j].q != params[0].q || params[j].pop_format != params[0].pop_format) return SSMD_EUNSUPPORTED;
         int kinds[8];
         const int nk = face_kinds(params[j], kinds);

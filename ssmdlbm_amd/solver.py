@@ -52,6 +52,14 @@ class Simulation:
         self.d.g = self._pops["ga"].data_ptr()
         self.d.g_next = self._pops["gb"].data_ptr()
         self.d.owned = 0
+        if int(self.p.closure) == abi.CLOSURE_LEARNED:
+            # eddy relaxation time, network inputs (8 bf16 per cell), packed network (zero = tau_t 0)
+            self.t["tau_t"] = z(n)
+            self.t["feat"] = z(n * abi.MLP_INPUTS, dtype=torch.int16)
+            self.t["mlp"] = z(int(self.lib.ssmd_mlp_packed_bytes()), dtype=torch.uint8)
+            self.d.tau_t = self.t["tau_t"].data_ptr()
+            self.d.feat = self.t["feat"].data_ptr()
+            self.d.mlp = self.t["mlp"].data_ptr()
         self.step_index = 0
         self._graph: Optional[C.c_void_p] = None
         torch.cuda.synchronize(self.device)  # zero fills above ran on the current stream
@@ -85,9 +93,13 @@ class Simulation:
         abi.check(self.lib.ssmd_launch_lattice(C.byref(self.p), C.byref(self.d), abi.STEP_MACROS if macros else 0,
                                                self.stream_handle), "lattice")
 
-    def launch_boundaries(self) -> None:
+    def launch_boundaries(self, macros: bool = False) -> None:
         abi.check(self.lib.ssmd_launch_boundaries(C.byref(self.p), C.byref(self.d), self.step_index, 0,
-                                                  self.stream_handle), "boundaries")
+                                                  abi.STEP_MACROS if macros else 0, self.stream_handle), "boundaries")
+
+    def closure_update(self) -> None:
+        """The LEARNED closure's network pass (the last launch of a step)."""
+        abi.check(self.lib.ssmd_closure_update(C.byref(self.p), C.byref(self.d), self.stream_handle), "closure")
 
     def swap(self) -> None:
         abi.check(self.lib.ssmd_swap(C.byref(self.d)), "swap")

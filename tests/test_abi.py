@@ -39,6 +39,9 @@ int main(void) {
   printf("%zu %zu %zu %zu\\n", offsetof(ssmd_params, visc_water), offsetof(ssmd_params, seed),
          offsetof(ssmd_params, water), offsetof(ssmd_params, oil));
   printf("%zu %zu %zu\\n", offsetof(ssmd_fields, f), offsetof(ssmd_fields, step_counter), offsetof(ssmd_fields, owned));
+  printf("%zu %zu %zu %zu\\n", offsetof(ssmd_params, closure), offsetof(ssmd_params, cs),
+         offsetof(ssmd_params, tau_scale), offsetof(ssmd_fields, mlp));
+  printf("%d %d %d\\n", SSMD_ABI_VERSION, SSMD_MLP_PARAMS, SSMD_CLOSURE_LEARNED);
   return 0;
 }
 """)
@@ -50,6 +53,9 @@ int main(void) {
     assert out[1] == [abi.Params.visc_water.offset, abi.Params.seed.offset, abi.Params.water.offset,
                       abi.Params.oil.offset]
     assert out[2] == [abi.Fields.f.offset, abi.Fields.step_counter.offset, abi.Fields.owned.offset]
+    assert out[3] == [abi.Params.closure.offset, abi.Params.cs.offset, abi.Params.tau_scale.offset,
+                      abi.Fields.mlp.offset]
+    assert out[4] == [abi.ABI_VERSION, abi.MLP_PARAMS, abi.CLOSURE_LEARNED]
 
 
 def test_reference_default_constants():
@@ -96,6 +102,10 @@ def test_validation_rejects_bad_parameters():
     b = p.copy()
     b.nx, b.ny, b.nz = 2048, 2048, 1100  # beyond 32-bit cell indices
     assert lib.ssmd_validate(b) == -2
+    b.nx, b.ny, b.nz = 1024, 1024, 1024  # 4 GiB float planes: beyond 32-bit byte offsets
+    assert lib.ssmd_validate(b) == -2
+    b.nz = 1020  # the largest cube-like box that fits
+    assert lib.ssmd_validate(b) == 0
     assert lib.ssmd_bytes(p, 1) == 19 * 4 * p.cells
     b = p.copy()
     b.pop_format = abi.POP_F16

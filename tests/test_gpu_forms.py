@@ -99,11 +99,12 @@ def test_vti_and_bin_output(gpu, tmp_path):
     raw = np.fromfile(tmp_path / "phi000003.bin", dtype=np.float32).reshape(p.shape)
     np.testing.assert_array_equal(raw, sim.field("phi").astype(np.float32))
     blob = (tmp_path / "step_000003.vti").read_bytes()
-    head = blob[:blob.index(b"_") + 1].decode()
+    marker = b'<AppendedData encoding="raw">\n'
+    off = blob.index(b"_", blob.index(marker) + len(marker)) + 1
+    head = blob[:off].decode()
     assert 'WholeExtent="0 19 0 17 0 23"' in head and 'Name="phi"' in head and 'Name="rho"' in head
     assert 'Name="missing"' not in head
     nbytes = p.cells * 4
-    off = blob.index(b"_") + 1
     assert struct.unpack("<I", blob[off:off + 4])[0] == nbytes
     np.testing.assert_array_equal(np.frombuffer(blob[off + 4:off + 4 + nbytes], dtype=np.float32), raw.reshape(-1))
     rho = np.frombuffer(blob[off + 8 + nbytes:off + 8 + 2 * nbytes], dtype=np.float32)
