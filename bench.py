@@ -71,6 +71,11 @@ def measure(case, steps: int, warmup: int, kernel_steps: int, tune: bool = True)
     torch.cuda.synchronize()
     times = sorted(e0.elapsed_time(e1) for e0, e1 in ev)
     k_ms = sum(times) / len(times)
+    # the state after all timed steps: one step writing every moment, then a finiteness check
+    sim.step(1, macros=True)
+    sim.synchronize()
+    finite = bool(torch.isfinite(sim.t["uz"]).all()) and bool(torch.isfinite(sim.t["phi"]).all())
+    max_uz = float(sim.t["uz"].abs().max())
     kb = cases.lattice_kernel_bytes_per_cell(case.q, case.pop_format)
     achieved = interior * kb / (k_ms * 1e-3) / 1e9
     step_bytes = cases.algorithmic_bytes_per_cell(case.q, case.pop_format)
@@ -88,6 +93,8 @@ def measure(case, steps: int, warmup: int, kernel_steps: int, tune: bool = True)
         "interior_cells": interior,
         "step_bytes_per_cell": step_bytes,
         "step_frac": mlups * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9),
+        "state_finite": finite,
+        "max_abs_uz": max_uz,
     }
 
 
@@ -305,6 +312,7 @@ def main() -> int:
             "timed_region": "HIP graph replay, one graph per step (phase, fused lattice, boundary kernels)",
             "lattice_tile": main_m["tile"],
             "mlups_cells": "all cells incl. ghost layer (reference convention)",
+            "state_finite_after_run": main_m["state_finite"],
         },
         "roofline": {
             "bound": "hbm",

@@ -157,8 +157,9 @@ const char *ssmd_error_string(int code);
 int ssmd_params_reference_default(ssmd_params *p, int32_t q);
 
 /* 0 if *p describes a case this build runs, else SSMD_EINVAL/SSMD_EUNSUPPORTED.
- * Size limit: 4 * (nx*ny*nz + nx*ny) < 2^32 (the kernels address one
- * population plane with 32-bit byte offsets), e.g. 1024 x 1024 x 1020. */
+ * Size limits: 4 * (nx*ny*nz + nx*ny) < 2^32 (the kernels address one
+ * population plane with 32-bit byte offsets), e.g. 1024 x 1024 x 1020, and
+ * at most 65536 cells along each axis. */
 int ssmd_validate(const ssmd_params *p);
 
 /* Bytes of one buffer: which = 0 scalar field, 1 f, 2 g. 0 on bad params. */
@@ -259,6 +260,36 @@ int ssmd_write_bin(const ssmd_params *p, const float *device_field, const char *
 int ssmd_write_vti(const ssmd_params *p, const char *dir, uint64_t step, const char *const *names, int32_t count);
 int ssmd_write_info(const ssmd_params *p, const char *dir, const char *id, const char *velocity_set, uint64_t nsteps,
                     uint64_t save_every);
+
+/* ---------------------------------------------------------------------------
+ * Derived statistics (reference src/derivedFields/, compiled out by default
+ * there, compile.sh:71-74).  A null pointer skips that field, as the
+ * reference's per-field checks do (timeAverage.cuh:52-78). */
+typedef struct ssmd_derived {
+    float *avg_phi, *avg_ux, *avg_uy, *avg_uz;                              /* timeAverage.cuh:37-82 */
+    float *avg_uxux, *avg_uyuy, *avg_uzuz, *avg_uxuy, *avg_uxuz, *avg_uyuz; /* reynoldsMoments.cuh:37-96 */
+    float *vort_x, *vort_y, *vort_z, *vort_mag;                             /* vorticityFields.cuh:37-105 */
+} ssmd_derived;
+
+/* Update after a step run with SSMD_STEP_MACROS (phi, rho, u current on
+ * every cell), where the reference launches them (main.cu:175):
+ *   averages  avg <- avg + (v - avg) / t with t = step + 1 (every cell);
+ *   vorticity central differences of u on interior cells, 0 on the ghost
+ *             layer, and its magnitude. */
+int ssmd_derived_update(const ssmd_params *p, const ssmd_fields *d, const ssmd_derived *s, uint64_t step,
+                        void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Checkpoint / restart (no counterpart in the reference).  The file holds the
+ * state one step hands to the next -- the populations the next step reads
+ * (f, g), phi (its ghost layer is boundary data), the moment fields and, for
+ * the LEARNED closure, tau_t -- with the step index and the parameters.
+ * ssmd_checkpoint_read initializes the fields (ssmd_init: ghost slots no
+ * kernel writes keep their t = 0 values) and then loads that state; continuing
+ * from `step` reproduces the uninterrupted run bit for bit.  Both calls
+ * synchronize `stream`. */
+int ssmd_checkpoint_write(const ssmd_params *p, const ssmd_fields *d, uint64_t step, const char *path, void *stream);
+int ssmd_checkpoint_read(const ssmd_params *p, ssmd_fields *d, uint64_t *step, const char *path, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Learned closure network (no counterpart in the reference).

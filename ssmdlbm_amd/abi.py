@@ -96,6 +96,12 @@ class Fields(C.Structure):
     ]
 
 
+class Derived(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "avg_phi", "avg_ux", "avg_uy", "avg_uz", "avg_uxux", "avg_uyuy", "avg_uzuz", "avg_uxuy", "avg_uxuz",
+        "avg_uyuz", "vort_x", "vort_y", "vort_z", "vort_mag")]
+
+
 # name -> (restype, argtypes); every function the header declares
 SIGNATURES = {
     "ssmd_abi_version": (C.c_int, []),
@@ -132,6 +138,11 @@ SIGNATURES = {
     "ssmd_write_bin": (C.c_int, [C.POINTER(Params), C.c_void_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_void_p]),
     "ssmd_write_vti": (C.c_int, [C.POINTER(Params), C.c_char_p, C.c_uint64, C.POINTER(C.c_char_p), C.c_int32]),
     "ssmd_write_info": (C.c_int, [C.POINTER(Params), C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64]),
+    "ssmd_derived_update": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.POINTER(Derived), C.c_uint64,
+                                      C.c_void_p]),
+    "ssmd_checkpoint_write": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_uint64, C.c_char_p, C.c_void_p]),
+    "ssmd_checkpoint_read": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.POINTER(C.c_uint64), C.c_char_p,
+                                       C.c_void_p]),
     "ssmd_mlp_packed_bytes": (C.c_size_t, []),
     "ssmd_mlp_pack": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "ssmd_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),

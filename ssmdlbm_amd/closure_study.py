@@ -48,7 +48,11 @@ class Geometry:
     ny: int = 128
     nz: int = 512
     q: int = 27
-    tau: float = 0.51
+    # Base relaxation time of the coarse grid.  The capillary-driven jets
+    # diverge at tau 0.51 and 0.52 (with or without Smagorinsky, coarse and
+    # fine alike); at 0.55 the fine twin (tau 0.6, sigma_f = 2 sigma) runs
+    # 20000 steps for sigma <= 0.03 and diverges for sigma >= 0.06.
+    tau: float = 0.55
     tau_g: float = 0.8
     width: float = 4.0
     diameter: int = 16
@@ -116,9 +120,13 @@ class Filter:
         self.weight = torch.empty(nc, device=device)
 
     def __call__(self, sim: Simulation) -> None:
+        # torch reads these buffers on its own stream: no filter pass may
+        # overlap a read of the previous one, and reads start after this one
+        torch.cuda.synchronize(sim.device)
         abi.check(self.lib.ssmd_sgs_samples(C.byref(self.fine), C.byref(sim.d), C.byref(self.coarse),
                                             self.work.data_ptr(), self.feat.data_ptr(), self.target.data_ptr(),
                                             self.weight.data_ptr(), sim.stream_handle), "ssmd_sgs_samples")
+        torch.cuda.synchronize(sim.device)
 
     def plane(self, k: int) -> torch.Tensor:
         """Filtered coarse plane k as (nz, ny, nx) (0 rho, 1-3 u, 4 phi, 5-10 S, 11-16 T)."""
@@ -243,7 +251,9 @@ def run_coarse(g: Geometry, jet: Jet, closure: int, net: Network | None, coarse_
         if m - 1 > sim.step_index:
             sim.graph_run(m - 1 - sim.step_index)
         sim.step(1, macros=True)
+        torch.cuda.synchronize(sim.device)  # the statistics read the fields on torch's stream
         stats.add(sim.t["uz"].view(p.shape), sim.t["phi"].view(p.shape))
+        torch.cuda.synchronize(sim.device)  # before the next steps overwrite them
     sim.synchronize()
     wall = time.perf_counter() - t0
     ok = bool(torch.isfinite(sim.t["uz"]).all())
@@ -312,9 +322,9 @@ def fit_scales(g: Geometry, jets: list[Jet], probe_steps: int) -> Scales:
 def train_main(args) -> int:
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
-    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q)
+    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q, tau=args.tau)
     rng = np.random.default_rng(args.seed)
-    jets = [Jet(float(rng.uniform(0.03, 0.08)), float(rng.uniform(0.02, 0.1)), int(args.seed * 100 + j))
+    jets = [Jet(float(rng.uniform(*args.u_range)), float(rng.uniform(*args.sigma_range)), int(args.seed * 100 + j))
             for j in range(args.runs)]
     log(f"training jets: {[asdict(j) for j in jets]}")
     scales = fit_scales(g, jets, max(args.steps // 4, 200))
@@ -381,7 +391,7 @@ def run_fine_scaled(g: Geometry, jet: Jet, snaps: list[int], scales: Scales, on_
 def eval_main(args) -> int:
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
-    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q)
+    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q, tau=args.tau)
     jet = Jet(args.u, args.sigma, args.seed)
     net = Network.load(args.net)
     z_range = (1, args.nz // 2)
@@ -419,8 +429,11 @@ def main(argv=None) -> int:
         s.add_argument("--q", type=int, default=27)
         s.add_argument("--steps", type=int, default=10000, help="coarse steps (fine runs take twice as many)")
         s.add_argument("--seed", type=int, default=5)
+        s.add_argument("--tau", type=float, default=Geometry.tau, help="coarse base relaxation time")
     t = sub.choices["train"]
     t.add_argument("--runs", type=int, default=2)
+    t.add_argument("--u-range", type=float, nargs=2, default=(0.03, 0.08), help="inlet u_z of the training jets")
+    t.add_argument("--sigma-range", type=float, nargs=2, default=(0.01, 0.03), help="sigma of the training jets")
     t.add_argument("--snapshots", type=int, default=6)
     t.add_argument("--per-snapshot", type=int, default=1 << 20)
     t.add_argument("--train-steps", type=int, default=3000)
@@ -429,7 +442,7 @@ def main(argv=None) -> int:
     e = sub.choices["eval"]
     e.add_argument("--net", required=True)
     e.add_argument("--u", type=float, default=0.05)
-    e.add_argument("--sigma", type=float, default=0.05)
+    e.add_argument("--sigma", type=float, default=0.02)
     e.add_argument("--every", type=int, default=50)
     e.add_argument("--kmax", type=int, default=32)
     args = ap.parse_args(argv)

@@ -6,10 +6,13 @@
 //                    ImageData step_<step:06>.vti, appended raw encoding, a
 //                    uint32 byte count before each array, re-reading the .bin files
 //   ssmd_write_info  host::generateSimulationInfoFile (functions/hostFunctions.cuh:47-87)
+//   ssmd_checkpoint_write / _read: restart files (no reference counterpart)
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <filesystem>
 #include <fstream>
 #include <iomanip>
@@ -26,6 +29,39 @@ std::string step6(uint64_t step) {
     ss << std::setw(6) << std::setfill('0') << step;
     return ss.str();
 }
+
+// Restart file: header, then the buffers below in order, raw.
+constexpr char kMagic[8] = {'S', 'S', 'M', 'D', 'C', 'K', 'P', '1'};
+struct CkptHeader {
+    char magic[8];
+    int32_t abi, params_bytes;
+    uint64_t step;
+    ssmd_params p;
+};
+
+struct Buf {
+    void *ptr;
+    size_t bytes;
+};
+
+// f, g as the next step reads them, phi, the 13 moment fields, tau_t.
+std::vector<Buf> state_buffers(const ssmd_params &p, const ssmd_fields &d) {
+    const size_t n = size_t(p.nx) * p.ny * p.nz;
+    const size_t sf = p.pop_format == SSMD_POP_F32 ? 4 : 2;
+    std::vector<Buf> b = {{d.f, n * size_t(p.q) * sf}, {d.g, n * 7 * 4}};
+    float *scal[] = {d.phi, d.rho, d.ux, d.uy, d.uz, d.pxx, d.pyy, d.pzz, d.pxy, d.pxz, d.pyz, d.fsx, d.fsy, d.fsz};
+    for (float *q : scal) b.push_back({q, n * 4});
+    if (p.closure == SSMD_CLOSURE_LEARNED) b.push_back({d.tau_t, n * 4});
+    return b;
+}
+
+// The parameters that fix the state's shape and meaning (not the launch shape).
+bool same_case(const ssmd_params &a, const ssmd_params &b) {
+    return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.q == b.q && a.pop_format == b.pop_format &&
+           a.closure == b.closure;
+}
+
+constexpr size_t kChunk = size_t(64) << 20;
 
 }  // namespace
 
@@ -102,6 +138,70 @@ int ssmd_write_vti(const ssmd_params *p, const char *dir, uint64_t step, const c
     v << "\n  </AppendedData>\n";
     v << "</VTKFile>\n";
     return v ? SSMD_OK : SSMD_EINVAL;
+}
+
+int ssmd_checkpoint_write(const ssmd_params *p, const ssmd_fields *d, uint64_t step, const char *path, void *stream) {
+    if (!p || !d || !path) return SSMD_EINVAL;
+    if (ssmd_validate(p) != 0) return SSMD_EINVAL;
+    const std::vector<Buf> bufs = state_buffers(*p, *d);
+    for (const Buf &b : bufs)
+        if (!b.ptr) return SSMD_ENOTREADY;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (const hipError_t e = hipStreamSynchronize(s); e != hipSuccess) return int(e);
+    std::ofstream f(path, std::ios::binary | std::ios::trunc);
+    if (!f) return SSMD_EINVAL;
+    CkptHeader h{};
+    std::memcpy(h.magic, kMagic, sizeof(kMagic));
+    h.abi = SSMD_ABI_VERSION;
+    h.params_bytes = int32_t(sizeof(ssmd_params));
+    h.step = step;
+    h.p = *p;
+    f.write(reinterpret_cast<const char *>(&h), sizeof(h));
+    std::vector<char> host(kChunk);
+    for (const Buf &b : bufs)
+        for (size_t off = 0; off < b.bytes; off += kChunk) {
+            const size_t len = std::min(kChunk, b.bytes - off);
+            hipError_t e = hipMemcpyAsync(host.data(), static_cast<const char *>(b.ptr) + off, len,
+                                          hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return int(e);
+            f.write(host.data(), std::streamsize(len));
+        }
+    return f ? SSMD_OK : SSMD_EINVAL;
+}
+
+int ssmd_checkpoint_read(const ssmd_params *p, ssmd_fields *d, uint64_t *step, const char *path, void *stream) {
+    if (!p || !d || !step || !path) return SSMD_EINVAL;
+    if (ssmd_validate(p) != 0) return SSMD_EINVAL;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return SSMD_EINVAL;
+    CkptHeader h{};
+    f.read(reinterpret_cast<char *>(&h), sizeof(h));
+    if (!f || std::memcmp(h.magic, kMagic, sizeof(kMagic)) != 0 || h.abi != SSMD_ABI_VERSION ||
+        h.params_bytes != int32_t(sizeof(ssmd_params)) || !same_case(h.p, *p))
+        return SSMD_EINVAL;
+    const std::vector<Buf> bufs = state_buffers(*p, *d);
+    for (const Buf &b : bufs)
+        if (!b.ptr) return SSMD_ENOTREADY;
+    // Ghost-cell slots that no kernel writes keep their t = 0 values in both
+    // buffers of each pair for the whole run: initialize, then overwrite the
+    // state the file carries.
+    if (const int rc = ssmd_init(p, d, stream); rc != 0) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (const hipError_t e = hipStreamSynchronize(s); e != hipSuccess) return int(e);
+    std::vector<char> host(kChunk);
+    for (const Buf &b : bufs)
+        for (size_t off = 0; off < b.bytes; off += kChunk) {
+            const size_t len = std::min(kChunk, b.bytes - off);
+            f.read(host.data(), std::streamsize(len));
+            if (!f) return SSMD_EINVAL;
+            hipError_t e = hipMemcpyAsync(static_cast<char *>(b.ptr) + off, host.data(), len, hipMemcpyHostToDevice,
+                                          s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return int(e);
+        }
+    *step = h.step;
+    return SSMD_OK;
 }
 
 int ssmd_write_info(const ssmd_params *p, const char *dir, const char *id, const char *velocity_set, uint64_t nsteps,

@@ -59,14 +59,9 @@ __device__ __forceinline__ bool is_interior(const KParams &k, int x, int y, int 
 
 // --------------------------------------------------------------------------
 // K1: phi = sum of the seven arriving phase populations (phaseField.cuh:31-52).
-__device__ __forceinline__ void phase_cell(const KParams &k, const float *__restrict__ g, float *__restrict__ phi,
-                                           uint32_t c) {
-    if (c >= k.N) return;
-    const int x = int(c % uint32_t(k.nx));
-    const uint32_t r = c / uint32_t(k.nx);
-    const int y = int(r % uint32_t(k.ny));
-    const int z = int(r / uint32_t(k.ny));
-    if (!is_interior(k, x, y, z)) return;
+__device__ __forceinline__ void phase_xyz(const KParams &k, const float *__restrict__ g, float *__restrict__ phi,
+                                          int x, int y, int z) {
+    const uint32_t c = cell_id(k, x, y, z);
     // source slots across a periodic face are read from their periodic image
     int sxp = 0, sxm = 0, syp = 0, sym = 0, szp = 0, szm = 0;
     if (k.wrap_x) {
@@ -96,9 +91,26 @@ __device__ __forceinline__ void phase_cell(const KParams &k, const float *__rest
     phi[c] = s;
 }
 
+__device__ __forceinline__ void phase_cell(const KParams &k, const float *__restrict__ g, float *__restrict__ phi,
+                                           uint32_t c) {
+    if (c >= k.N) return;
+    const int x = int(c % uint32_t(k.nx));
+    const uint32_t r = c / uint32_t(k.nx);
+    const int y = int(r % uint32_t(k.ny));
+    const int z = int(r / uint32_t(k.ny));
+    if (!is_interior(k, x, y, z)) return;
+    phase_xyz(k, g, phi, x, y, z);
+}
+
 #ifdef SSMD_STEP_KERNELS
+// Grid (ceil(nx / 64), ceil((ny - 2) / 4), nz - 2) of 64 x 4 blocks over the
+// interior rows: coordinates come from the launch, not from divisions.
 __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi) {
-    phase_cell(k, g, phi, blockIdx.x * 256u + threadIdx.x);
+    const int x = int(blockIdx.x) * 64 + int(threadIdx.x);
+    const int y = 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
+    const int z = 1 + int(blockIdx.z);
+    if (x < 1 || x > k.nx - 2 || y > k.ny - 2) return;
+    phase_xyz(k, g, phi, x, y, z);
 }
 #endif  // SSMD_STEP_KERNELS
 
@@ -281,7 +293,12 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             if constexpr (d.z < 0) shift += zsm;
             const char *base = reinterpret_cast<const char *>(fin + shift);
             const uint32_t off = d.x > 0 ? op : (d.x < 0 ? om : o0);
-            f[i] = P::load(*reinterpret_cast<const PT *>(base + off));
+            const PT *src = reinterpret_cast<const PT *>(base + off);
+            // D3Q27 reads faster with non-temporal loads, D3Q19 slower (measured)
+            if constexpr (Q == 27)
+                f[i] = P::load(__builtin_nontemporal_load(src));
+            else
+                f[i] = P::load(*src);
         });
     };
     // The populations of plane z+1 are in flight while plane z collides:
@@ -407,17 +424,20 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         const float omco = 1.0f - fast_rcp(k.closure == SSMD_CLOSURE_NONE
                                                ? 0.5f + 3.0f * nu
                                                : closure_tau(k, mac, m, c, ph, nc.w, 0.5f + 3.0f * nu));
-        // uniform plane base + shared 32-bit byte offset for every store
+        // uniform plane base + shared 32-bit byte offset for every store;
+        // non-temporal: the next step reads these from HBM, nothing here does
         size_t Ns = N0;
         asm volatile("" : "+s"(Ns));
         const uint32_t so = c * uint32_t(sizeof(PT));
         collide<Q>(m, omco, kx, ky, kz, [&](int i, float v) {
-            *reinterpret_cast<PT *>(reinterpret_cast<char *>(fout + size_t(i) * Ns) + so) = P::store(v);
+            PT *dst = reinterpret_cast<PT *>(reinterpret_cast<char *>(fout + size_t(i) * Ns) + so);
+            __builtin_nontemporal_store(P::store(v), dst);
         });
         const float sharp = k.gamma * ph * (1.0f - ph);
         const uint32_t go = c * 4u;
         phase_collide(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) {
-            *reinterpret_cast<float *>(reinterpret_cast<char *>(gout + size_t(kk) * Ns) + go) = v;
+            float *dst = reinterpret_cast<float *>(reinterpret_cast<char *>(gout + size_t(kk) * Ns) + go);
+            __builtin_nontemporal_store(v, dst);
         });
 
         const bool face = fx_lo || fx_hi || fy_lo || fy_hi || fz_lo || fz_hi;
@@ -481,6 +501,49 @@ __device__ __forceinline__ int find_slot(const JetSlot *jets, int njets, uint32_
 }
 
 #ifdef SSMD_STEP_KERNELS
+// Derived statistics of one step (reference derivedFields/operators/:
+// timeAverage.cuh:37-82, reynoldsMoments.cuh:37-96, vorticityFields.cuh:37-105).
+struct Derived {
+    float *avg[4];   // phi, ux, uy, uz
+    float *rey[6];   // uxux uyuy uzuz uxuy uxuz uyuz
+    float *vort[4];  // x, y, z, magnitude
+};
+
+__global__ void __launch_bounds__(256) k_derived(KParams k, const float *__restrict__ phi,
+                                                 const float *__restrict__ ux, const float *__restrict__ uy,
+                                                 const float *__restrict__ uz, Derived s, float t) {
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= k.N) return;
+    auto upd = [t](float *a, uint32_t i, float v) { a[i] = a[i] + (v - a[i]) / t; };
+    const float vx = ux[c], vy = uy[c], vz = uz[c];
+    if (s.avg[0]) upd(s.avg[0], c, phi[c]);
+    if (s.avg[1]) upd(s.avg[1], c, vx);
+    if (s.avg[2]) upd(s.avg[2], c, vy);
+    if (s.avg[3]) upd(s.avg[3], c, vz);
+    if (s.rey[0]) upd(s.rey[0], c, vx * vx);
+    if (s.rey[1]) upd(s.rey[1], c, vy * vy);
+    if (s.rey[2]) upd(s.rey[2], c, vz * vz);
+    if (s.rey[3]) upd(s.rey[3], c, vx * vy);
+    if (s.rey[4]) upd(s.rey[4], c, vx * vz);
+    if (s.rey[5]) upd(s.rey[5], c, vy * vz);
+    if (s.vort[0] || s.vort[1] || s.vort[2] || s.vort[3]) {
+        const int x = int(c % uint32_t(k.nx)), y = int(c / uint32_t(k.nx) % uint32_t(k.ny));
+        const int z = int(c / k.sxy);
+        float wx = 0.f, wy = 0.f, wz = 0.f;
+        if (is_interior(k, x, y, z)) {
+            const uint32_t xp = c + 1, xm = c - 1, yp = c + uint32_t(k.nx), ym = c - uint32_t(k.nx);
+            const uint32_t zp = c + k.sxy, zm = c - k.sxy;
+            wx = 0.5f * (uz[yp] - uz[ym]) - 0.5f * (uy[zp] - uy[zm]);
+            wy = 0.5f * (ux[zp] - ux[zm]) - 0.5f * (uz[xp] - uz[xm]);
+            wz = 0.5f * (uy[xp] - uy[xm]) - 0.5f * (ux[yp] - ux[ym]);
+        }
+        if (s.vort[0]) s.vort[0][c] = wx;
+        if (s.vort[1]) s.vort[1][c] = wy;
+        if (s.vort[2]) s.vort[2][c] = wz;
+        if (s.vort[3]) s.vort[3][c] = __builtin_sqrtf(wx * wx + wy * wy + wz * wz);
+    }
+}
+
 // K1 for every jet: one 256-thread block per 256 cells of a jet.
 __global__ void __launch_bounds__(256) k_phase_batch(const JetSlot *__restrict__ jets, int njets) {
     const int j = find_slot(jets, njets, blockIdx.x, 1);

@@ -160,7 +160,8 @@ int face_axis_host(int kind) {
 
 int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
     const KParams k = make_kparams(p);
-    hipLaunchKernelGGL(k_phase, dim3((k.N + 255) / 256), dim3(256), 0, s, k, d.g, d.phi);
+    const dim3 grid(unsigned(p.nx + 63) / 64, unsigned(p.ny - 2 + 3) / 4, unsigned(p.nz - 2));
+    hipLaunchKernelGGL(k_phase, grid, dim3(64, 4), 0, s, k, d.g, d.phi);
     return herr(hipGetLastError());
 }
 
@@ -313,6 +314,8 @@ int ssmd_validate(const ssmd_params *p) {
     // offsets (4-byte elements at most), one plane of slack for the pulls.
     const uint64_t n = uint64_t(p->nx) * p->ny * p->nz;
     if (4 * (n + uint64_t(p->nx) * p->ny) >= (uint64_t(1) << 32)) return SSMD_EUNSUPPORTED;
+    // launch grids put rows and planes on grid y / z (at most 65535 blocks)
+    if (p->nx > 65536 || p->ny > 65536 || p->nz > 65536) return SSMD_EUNSUPPORTED;
     for (int w : {p->wrap_x, p->wrap_y, p->wrap_z})
         if (w < SSMD_WRAP_NONE || w > SSMD_WRAP_TRUE) return SSMD_EINVAL;
     if (p->wrap_y && (p->water.enabled || p->outflow_y)) return SSMD_EINVAL;
@@ -449,6 +452,32 @@ int ssmd_step(const ssmd_params *p, ssmd_fields *d, uint64_t step, uint32_t flag
     SSMD_TRY(step_dispatch(*p, *d, nullptr, step, flags, static_cast<hipStream_t>(stream)));
     swap_buffers(d);
     return SSMD_OK;
+}
+
+int ssmd_derived_update(const ssmd_params *p, const ssmd_fields *d, const ssmd_derived *s, uint64_t step,
+                        void *stream) {
+    SSMD_TRY(ssmd_validate(p));
+    if (!s) return SSMD_EINVAL;
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
+    const KParams k = make_kparams(*p);
+    Derived v{};
+    v.avg[0] = s->avg_phi;
+    v.avg[1] = s->avg_ux;
+    v.avg[2] = s->avg_uy;
+    v.avg[3] = s->avg_uz;
+    v.rey[0] = s->avg_uxux;
+    v.rey[1] = s->avg_uyuy;
+    v.rey[2] = s->avg_uzuz;
+    v.rey[3] = s->avg_uxuy;
+    v.rey[4] = s->avg_uxuz;
+    v.rey[5] = s->avg_uyuz;
+    v.vort[0] = s->vort_x;
+    v.vort[1] = s->vort_y;
+    v.vort[2] = s->vort_z;
+    v.vort[3] = s->vort_mag;
+    hipLaunchKernelGGL(k_derived, dim3((k.N + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream), k, d->phi,
+                       d->ux, d->uy, d->uz, v, float(step + 1));
+    return herr(hipGetLastError());
 }
 
 int ssmd_closure_update(const ssmd_params *p, const ssmd_fields *d, void *stream) {

@@ -116,6 +116,19 @@ class Simulation:
                   "ssmd_graph_launch")
         self.step_index += n
 
+    # -- restart ------------------------------------------------------------
+    def checkpoint(self, path: str) -> None:
+        """Write the state the next step needs (ssmd_checkpoint_write)."""
+        abi.check(self.lib.ssmd_checkpoint_write(C.byref(self.p), C.byref(self.d), self.step_index,
+                                                 str(path).encode(), self.stream_handle), "checkpoint_write")
+
+    def restore(self, path: str) -> None:
+        """Load a checkpoint of the same case; the step index comes with it."""
+        step = C.c_uint64()
+        abi.check(self.lib.ssmd_checkpoint_read(C.byref(self.p), C.byref(self.d), C.byref(step), str(path).encode(),
+                                                self.stream_handle), "checkpoint_read")
+        self.step_index = int(step.value)
+
     def tune(self, reps: int = 20) -> tuple[int, int]:
         """Autotune the lattice tile shape and z-chunk; the choice is kept in self.p."""
         t, lz = C.c_int32(), C.c_int32()
@@ -307,3 +320,32 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+DERIVED_FIELDS = tuple(n for n, _ in abi.Derived._fields_)
+
+
+class DerivedStats:
+    """The reference's derived fields (derivedFields/) for one Simulation:
+    running means of phi and u, Reynolds moments <u_i u_j>, vorticity."""
+
+    def __init__(self, sim: Simulation, names: Iterable[str] = DERIVED_FIELDS):
+        self.sim = sim
+        self.t: Dict[str, torch.Tensor] = {}
+        self.s = abi.Derived()
+        for n in names:
+            if n not in DERIVED_FIELDS:
+                raise ValueError(f"unknown derived field {n}")
+            self.t[n] = torch.zeros(sim.p.cells, dtype=torch.float32, device=sim.device)
+            setattr(self.s, n, self.t[n].data_ptr())
+        torch.cuda.synchronize(sim.device)
+
+    def update(self) -> None:
+        """After a step run with macros=True: the statistics of that step."""
+        sim = self.sim
+        abi.check(sim.lib.ssmd_derived_update(C.byref(sim.p), C.byref(sim.d), C.byref(self.s), sim.step_index - 1,
+                                              sim.stream_handle), "derived_update")
+
+    def field(self, name: str) -> np.ndarray:
+        self.sim.synchronize()
+        return self.t[name].double().cpu().numpy().reshape(self.sim.p.shape)

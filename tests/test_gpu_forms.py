@@ -4,6 +4,7 @@ import struct
 
 import numpy as np
 import pytest
+import torch
 
 from ssmdlbm_amd import abi, cases
 from ssmdlbm_amd.solver import Ensemble, Simulation
@@ -110,3 +111,94 @@ def test_vti_and_bin_output(gpu, tmp_path):
     rho = np.frombuffer(blob[off + 8 + nbytes:off + 8 + 2 * nbytes], dtype=np.float32)
     np.testing.assert_array_equal(rho, sim.field("rho").astype(np.float32).reshape(-1))
     assert blob.endswith(b"\n  </AppendedData>\n</VTKFile>\n")
+
+
+# -- restart -------------------------------------------------------------------
+@pytest.mark.parametrize("closure", [abi.CLOSURE_NONE, abi.CLOSURE_LEARNED])
+def test_checkpoint_resume_is_bitwise(gpu, tmp_path, closure):
+    from ssmdlbm_amd.closure import Network, Scales, init_params
+    p = cases.JET_D3Q19_SMALL.with_(nx=24, ny=22, nz=30, diameter=8).params()
+    p.closure = closure
+    net = None
+    if closure == abi.CLOSURE_LEARNED:
+        p.strain_scale, p.grad_scale, p.tau_scale = 200.0, 4.0, 0.01
+        net = Network(init_params(3), scales=Scales(200.0, 4.0, 0.01))
+
+    def fresh():
+        s = Simulation(p)
+        if net is not None:
+            net.load_into(s)
+        s.init()
+        return s
+
+    a = fresh()
+    a.step(30)
+    a.checkpoint(tmp_path / "state.ckpt")
+    a.graph_run(20)
+    a.step(1, macros=True)
+    b = fresh()
+    b.step(3)  # the restore must not depend on what the fields held before
+    b.restore(tmp_path / "state.ckpt")
+    assert b.step_index == 30
+    b.graph_run(20)
+    b.step(1, macros=True)
+    for name in ("rho", "ux", "uy", "uz", "phi") + (("tau_t",) if net is not None else ()):
+        assert np.array_equal(a.field(name), b.field(name)), name
+    assert np.array_equal(a.pops_source("f"), b.pops_source("f"))
+    assert np.array_equal(a.pops_source("g"), b.pops_source("g"))
+
+
+def test_checkpoint_rejects_another_case(gpu, tmp_path):
+    p = cases.JET_D3Q19_SMALL.with_(nx=24, ny=22, nz=30, diameter=8).params()
+    a = Simulation(p)
+    a.init()
+    a.checkpoint(tmp_path / "s.ckpt")
+    q = p.copy()
+    q.nz = 32
+    b = Simulation(q)
+    with pytest.raises(abi.SsmdError):
+        b.restore(tmp_path / "s.ckpt")
+
+
+# -- derived statistics ---------------------------------------------------------
+def test_derived_statistics_closed_forms(gpu):
+    """Linear velocity u = s_t (a y, b z, c x) + (0, 0, w) and phi = s_t x over
+    t = 1..5 steps: the running means are mean(s_t) times the fields, the
+    Reynolds moments mean(s_t^2)-weighted products, and the vorticity of the
+    last step s_5 (-b, -c, -a) on interior cells, 0 on the ghost layer."""
+    from ssmdlbm_amd.solver import DerivedStats
+    p = cases.JET_D3Q19_SMALL.with_(nx=14, ny=12, nz=10, diameter=4).params()
+    sim = Simulation(p)
+    sim.init()
+    names = ("avg_phi", "avg_ux", "avg_uz", "avg_uxux", "avg_uxuz", "avg_uyuz", "vort_x", "vort_y", "vort_z",
+             "vort_mag")
+    ds = DerivedStats(sim, names)
+    nz, ny, nx = p.shape
+    Z, Y, X = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    a, b, c, w = 1e-3, 2e-3, -1.5e-3, 0.02
+    svals = [1.0, 0.5, 1.25, 0.75, 1.1]
+    for t, s in enumerate(svals):
+        fields = {"ux": s * a * Y, "uy": s * b * Z, "uz": s * c * X + w, "phi": s * X.astype(float)}
+        for k, v in fields.items():
+            sim.t[k].copy_(torch.tensor(v.reshape(-1), dtype=torch.float32))
+        torch.cuda.synchronize()
+        sim.step_index = t + 1  # the statistics of step t
+        ds.update()
+    m1, m2 = np.mean(svals), np.mean(np.square(svals))
+    ux, uy, uz = a * Y, b * Z, c * X
+    np.testing.assert_allclose(ds.field("avg_phi"), m1 * X, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(ds.field("avg_ux"), m1 * ux, rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(ds.field("avg_uz"), m1 * uz + w, rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(ds.field("avg_uxux"), m2 * ux * ux, rtol=1e-5, atol=1e-12)
+    expect_uxuz = np.mean([s * ux * (s * uz + w) for s in svals], axis=0)
+    np.testing.assert_allclose(ds.field("avg_uxuz"), expect_uxuz, rtol=1e-5, atol=1e-11)
+    np.testing.assert_allclose(ds.field("avg_uyuz"), np.mean([s * uy * (s * uz + w) for s in svals], axis=0),
+                               rtol=1e-5, atol=1e-11)
+    inner = np.zeros(p.shape, dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    s = svals[-1]
+    for name, val in (("vort_x", -s * b), ("vort_y", -s * c), ("vort_z", -s * a)):
+        f = ds.field(name)
+        np.testing.assert_allclose(f[inner], val, rtol=1e-4)
+        assert not f[~inner].any(), name
+    np.testing.assert_allclose(ds.field("vort_mag")[inner], s * np.sqrt(a * a + b * b + c * c), rtol=1e-4)
