@@ -98,6 +98,40 @@ def measure(case, steps: int, warmup: int, kernel_steps: int, tune: bool = True)
     }
 
 
+class ReferenceCase:
+    """The reference's compile-time default case (constants.cuh:70-101):
+    128x256x256, water jet on y = 0 and oil jet on z = 0, outflow y and z."""
+
+    def __init__(self, q: int, pop_format: int):
+        from ssmdlbm_amd import abi, cases
+
+        self.q, self.pop_format = q, pop_format
+        self.name = f"reference_default_d3q{q}_{['f32', 'f16', 'bf16'][pop_format]}_128x256x256"
+        self._p = cases.reference_default(q)
+        self._p.pop_format = pop_format
+        self.cells = self._p.cells
+        self._abi = abi
+
+    def params(self):
+        return self._p.copy()
+
+
+def measure_reference_cases(steps: int, warmup: int):
+    """MLUPS of the reference's own default case, beside its published
+    RTX 3050 figures (README.md:35-37: 440 D3Q19, 377 D3Q27)."""
+    from ssmdlbm_amd import abi
+
+    out = {}
+    for q in (19, 27):
+        for fmt in (abi.POP_F32, abi.POP_F16):
+            c = ReferenceCase(q, fmt)
+            m = measure(c, steps, warmup, 20)
+            out[c.name] = {"mlups": round(m["mlups"], 1), "kernel_ms": round(m["kernel_ms"], 4),
+                           "state_finite": m["state_finite"]}
+    out["published_rtx3050_mlups"] = {"d3q19": 440, "d3q27": 377}
+    return out
+
+
 def measure_ensemble(steps: int, warmup: int, n_jets: int = 32):
     """The batching form: a seeded ensemble of jets advanced by one launch per
     kernel kind per step (graph-replayed, autotuned tile), against the same
@@ -281,6 +315,8 @@ def main() -> int:
         forms["fp16_over_fp32_mlups"] = round(a / b, 4)
         forms["ensemble_32_jets_d3q19"] = measure_ensemble(args.form_steps, 50)
         log(f"ensemble: {forms['ensemble_32_jets_d3q19']}")
+        forms["reference_default_case"] = measure_reference_cases(max(args.form_steps, 500), 200)
+        log(f"reference default case: {forms['reference_default_case']}")
 
     cpu = None
     if not args.no_cpu_baseline:

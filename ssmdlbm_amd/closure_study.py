@@ -6,8 +6,8 @@ scaling (acoustic: dx and dt halve on the fine grid, lattice velocities
 agree):
 
   coarse  nx x ny x nz, tau_c, sigma_c, inlet diameter D, centre c
-  fine    2(n-2)+2.. cells per axis, tau_f = 1/2 + 2 (tau_c - 1/2),
-          sigma_f = 2 sigma_c, diameter 2D, centre 2c - 1/2
+  fine    2n cells per axis, tau_f = 1/2 + 2 (tau_c - 1/2), sigma_f =
+          2 sigma_c, interface width 2W, diameter 2D, centre 2c - 1/2
 
 so coarse interior cell i covers fine cells 2i-1, 2i (the box filter of
 ssmd_sgs_samples) and one coarse step is two fine steps.
@@ -94,6 +94,9 @@ def fine_params(g: Geometry, jet: Jet) -> abi.Params:
     p.nx, p.ny, p.nz = 2 * g.nx, 2 * g.ny, 2 * g.nz
     p.visc_water = p.visc_oil = viscosity_from_tau(0.5 + 2.0 * (g.tau - 0.5))
     p.sigma = 2.0 * jet.sigma
+    # the same physical interface width: twice the cells (the sharpening
+    # coefficient halves; the D3Q7 mobility is fixed in lattice units)
+    p.gamma = gamma_from_tau_g(g.tau_g, 2.0 * g.width)
     p.oil.center_a, p.oil.center_b = 2.0 * c.oil.center_a - 0.5, 2.0 * c.oil.center_b - 0.5
     r = g.diameter  # 2 * (D // 2) with D even
     p.oil.r2 = float(r * r)

@@ -345,3 +345,27 @@ def test_learned_closure_beats_no_closure_and_smagorinsky_a_priori(gpu):
           f"{x.shape[0]} training samples, target mean {float((w * t).mean()):.3f}")
     assert learned < 0.9 * none
     assert learned < 0.9 * smag_err
+
+
+def test_shipped_network_runs_a_learned_jet(gpu):
+    """The trained network in ssmdlbm_amd/data: loads with its scales,
+    installs into a D3Q27 LEARNED jet, and the run stays finite with
+    tau_t >= 0 (the closure clamps the network output at 0)."""
+    from pathlib import Path
+    import ssmdlbm_amd
+    from ssmdlbm_amd.closure_study import Geometry, Jet, coarse_params
+    path = Path(ssmdlbm_amd.__file__).parent / "data" / "closure_jet_d3q27"
+    net = Network.load(path)
+    assert net.scales.tau_scale > 0 and net.scales.strain_scale > 0
+    g = Geometry(nx=48, ny=48, nz=96, q=27, diameter=12)
+    p = coarse_params(g, Jet(0.05, 0.02, 9), abi.CLOSURE_LEARNED, net.scales)
+    sim = Simulation(p)
+    net.load_into(sim)
+    sim.init()
+    sim.graph_run(400)
+    sim.step(1, macros=True)
+    sim.synchronize()
+    tau_t = sim.field("tau_t")
+    assert np.isfinite(sim.field("uz")).all() and np.isfinite(tau_t).all()
+    assert tau_t.min() >= 0.0
+    print(f"shipped network: tau_t mean {tau_t.mean():.3e}, max {tau_t.max():.3e}")

@@ -169,6 +169,28 @@ def test_jet_d3q19_64x64x128_1000_steps(gpu):
             assert errs[k] <= max(2 * budget[k], 1e-6), (s, k, errs[k], budget[k])
 
 
+@pytest.mark.parametrize("fmt", [abi.POP_F16, abi.POP_BF16])
+def test_storage_forms_jet_64x64x128_1000_steps(gpu, fmt):
+    """The precision form's error statement: f16 / bf16 population storage on
+    the 64x64x128 jet over 1000 steps, per field against the FP64 oracle.
+    Bound: within 3x (+1e-6) of the oracle run with the same storage rounding
+    in FP32 arithmetic, i.e. the GPU adds nothing beyond the storage format."""
+    base = cases.JET_D3Q19_SMALL.params()
+    p = base.copy()
+    p.pop_format = fmt
+    sim = Simulation(p)
+    sim.init()
+    sim.graph_run(999)
+    sim.step(1, macros=True)
+    o64 = run_oracle(base, 1000)
+    o16 = run_oracle(p, 1000, fp64=False)
+    errs = {n: rel_l2(sim.field(n), o64.field(n)) for n in MACRO}
+    oerr = {n: rel_l2(o16.field(n), o64.field(n)) for n in MACRO}
+    print(f"storage {fmt}, 1000 steps, GPU vs FP64: {errs}; FP32-arithmetic oracle with the same storage: {oerr}")
+    for n in MACRO:
+        assert errs[n] <= 3 * oerr[n] + 1e-6, (n, errs[n], oerr[n])
+
+
 # -- wraps -----------------------------------------------------------------------
 @pytest.mark.parametrize("q", [19, 27])
 def test_true_periodic_wrap(gpu, q):
