@@ -105,11 +105,20 @@ __device__ __forceinline__ void phase_cell(const KParams &k, const float *__rest
 #ifdef SSMD_STEP_KERNELS
 // Grid (ceil(nx / 64), ceil((ny - 2) / 4), nz - 2) of 64 x 4 blocks over the
 // interior rows: coordinates come from the launch, not from divisions.
+// The ghost-column lanes (x = 0, nx-1) store back the phi they hold (boundary
+// data of the previous step) so that every phi line is written whole.
 __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi) {
     const int x = int(blockIdx.x) * 64 + int(threadIdx.x);
     const int y = 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
     const int z = 1 + int(blockIdx.z);
-    if (x < 1 || x > k.nx - 2 || y > k.ny - 2) return;
+    if (x > k.nx - 1 || y > k.ny - 2) return;
+    if (x == 0 || x == k.nx - 1) {
+        const uint32_t c = cell_id(k, x, y, z);
+        float v = phi[c];
+        asm volatile("" : "+v"(v));  // an opaque value: the store is not elided
+        phi[c] = v;
+        return;
+    }
     phase_xyz(k, g, phi, x, y, z);
 }
 #endif  // SSMD_STEP_KERNELS
