@@ -78,7 +78,8 @@ def measure(case, steps: int, warmup: int, kernel_steps: int, tune: bool = True)
     max_uz = float(sim.t["uz"].abs().max())
     kb = cases.lattice_kernel_bytes_per_cell(case.q, case.pop_format)
     achieved = interior * kb / (k_ms * 1e-3) / 1e9
-    step_bytes = cases.algorithmic_bytes_per_cell(case.q, case.pop_format)
+    b_alg = cases.algorithmic_bytes_per_cell(case.q, case.pop_format)
+    step_bytes = cases.step_bytes_per_cell(case.q, case.pop_format)
     sim.close()
     del sim
     torch.cuda.empty_cache()
@@ -92,7 +93,9 @@ def measure(case, steps: int, warmup: int, kernel_steps: int, tune: bool = True)
         "kernel_bytes_per_cell": kb,
         "interior_cells": interior,
         "step_bytes_per_cell": step_bytes,
-        "step_frac": mlups * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9),
+        "reference_step_bytes_per_cell": b_alg,
+        "step_frac": mlups * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9),
+        "step_frac_own_bytes": mlups * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9),
         "state_finite": finite,
         "max_abs_uz": max_uz,
     }
@@ -361,6 +364,7 @@ def main() -> int:
             "kernel_ms": round(main_m["kernel_ms"], 5),
             "algorithmic_bytes_per_launch": main_m["interior_cells"] * kb,
             "step_frac_at_B_alg": round(main_m["step_frac"], 4),
+            "step_frac_at_own_step_bytes": round(main_m["step_frac_own_bytes"], 4),
             "device_copy_gbs": round(copy_gbs, 1),
             "frac_of_device_copy": round(main_m["kernel_gbs"] / copy_gbs, 4),
         },

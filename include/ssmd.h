@@ -28,7 +28,13 @@
  *     ("pull"): slot f[i][s] holds the post-collision population that leaves
  *     cell s along c_i, so the population arriving at cell a along c_i is
  *     f[i][a - c_i].  Ghost-cell slots hold what the boundaries emit.
- *     Two buffer pairs ping-pong; ssmd_step swaps f<->f_next, g<->g_next.
+ *   - The D3Q7 phase populations are stored in moment form: g holds 4
+ *     planes (g_phi, o_x, o_y, o_z) per source cell s, and the population
+ *     leaving s along c_k is w_k (g_phi[s] + c_k . o[s]) (w_0 = 1/4, else
+ *     1/8).  The collision emits exactly this family (o = 4 phi u +
+ *     gamma phi (1-phi) n), so the arriving sum is the same FP32 sum as
+ *     with seven population planes, with 16 instead of 28 bytes per cell.
+ *   - Two buffer pairs ping-pong; ssmd_step swaps f<->f_next, g<->g_next.
  */
 #ifndef SSMD_H
 #define SSMD_H
@@ -40,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SSMD_ABI_VERSION 2
+#define SSMD_ABI_VERSION 3
 
 #define SSMD_OK 0
 #define SSMD_EINVAL (-1)       /* bad argument or inconsistent parameters */
@@ -131,7 +137,7 @@ typedef struct ssmd_fields {
     float *normx, *normy, *normz, *ind;
     float *fsx, *fsy, *fsz;
     void *f;          /* populations the next step reads (pop_format elements) */
-    float *g;         /* D3Q7 phase populations the next step reads */
+    float *g;         /* D3Q7 phase populations the next step reads (4 moment planes) */
     void *f_next;     /* buffers the next step writes */
     float *g_next;
     uint64_t *step_counter; /* device word: inlet-noise step index for graph replays */
@@ -162,7 +168,7 @@ int ssmd_params_reference_default(ssmd_params *p, int32_t q);
  * at most 65536 cells along each axis. */
 int ssmd_validate(const ssmd_params *p);
 
-/* Bytes of one buffer: which = 0 scalar field, 1 f, 2 g. 0 on bad params. */
+/* Bytes of one buffer: which = 0 scalar field, 1 f, 2 g (4 planes). 0 on bad params. */
 size_t ssmd_bytes(const ssmd_params *p, int32_t which);
 
 /* Allocate and zero every buffer (FieldAllocate semantics, hipMalloc). */

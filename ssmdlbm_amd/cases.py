@@ -131,15 +131,23 @@ def closed_box(n: int, q: int = 19, tau: float = 0.8, sigma: float = 0.01, gamma
 
 
 def algorithmic_bytes_per_cell(q: int, pop_format: int) -> int:
-    """Bytes one cell update must move: f read+write, g read+write, phi once
+    """Bytes one cell update of the reference's schedule moves (the whole-step
+    yardstick): f read+write, seven g populations read+write, phi once
     written and once read = 2 Q s_f + 2*7*4 + 2*4."""
     return 2 * q * abi.POP_BYTES[pop_format] + 2 * 7 * 4 + 2 * 4
 
 
+def step_bytes_per_cell(q: int, pop_format: int) -> int:
+    """Bytes one cell update of this library's step moves: f read+write, the
+    four g moment planes written (lattice kernel) and read (phase kernel), phi
+    written and read = 2 Q s_f + 2*4*4 + 2*4."""
+    return 2 * q * abi.POP_BYTES[pop_format] + 2 * 4 * 4 + 2 * 4
+
+
 def lattice_kernel_bytes_per_cell(q: int, pop_format: int) -> int:
     """Algorithmic bytes of the fused lattice kernel alone: f read + f write,
-    g write, phi read."""
-    return 2 * q * abi.POP_BYTES[pop_format] + 7 * 4 + 4
+    g moments (4 floats) written, phi read."""
+    return 2 * q * abi.POP_BYTES[pop_format] + 4 * 4 + 4
 
 
 def ensemble(n_jets: int = 32, seed: int = 4, q: int = 19, pop_format: int = abi.POP_F32) -> list[JetCase]:

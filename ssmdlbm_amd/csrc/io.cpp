@@ -48,7 +48,7 @@ struct Buf {
 std::vector<Buf> state_buffers(const ssmd_params &p, const ssmd_fields &d) {
     const size_t n = size_t(p.nx) * p.ny * p.nz;
     const size_t sf = p.pop_format == SSMD_POP_F32 ? 4 : 2;
-    std::vector<Buf> b = {{d.f, n * size_t(p.q) * sf}, {d.g, n * 7 * 4}};
+    std::vector<Buf> b = {{d.f, n * size_t(p.q) * sf}, {d.g, n * 4 * 4}};
     float *scal[] = {d.phi, d.rho, d.ux, d.uy, d.uz, d.pxx, d.pyy, d.pzz, d.pxy, d.pxz, d.pyz, d.fsx, d.fsy, d.fsz};
     for (float *q : scal) b.push_back({q, n * 4});
     if (p.closure == SSMD_CLOSURE_LEARNED) b.push_back({d.tau_t, n * 4});

@@ -3,13 +3,14 @@
 // Data layout in HBM (one field set):
 //   scalar fields  float[N], idx = x + nx*(y + ny*z)
 //   f, f_next      Q planes of N pop elements (f32 / f16 / bf16 deviations)
-//   g, g_next      7 planes of N floats
+//   g, g_next      4 planes of N floats: the D3Q7 phase populations in
+//                  moment form (g_phi, o_x, o_y, o_z), see phase_xyz
 // Populations are source-indexed ("pull"): slot [i][s] holds what leaves
 // cell s along c_i; the population arriving at cell a along c_i is slot
-// [i][a - c_i].  Every element is read once and written once per step.
+// [i][a - c_i].  Every f element is read once and written once per step.
 //
 // Per step:
-//   k_phase    phi(x) = sum_k g[k][x - c_k] on interior cells         (K1)
+//   k_phase    phi(x) = sum_k g_k(x - c_k) on interior cells          (K1)
 //   k_lattice  fused phase::computeForces + lbm::computeMoments +
 //              lbm::streamCollide: LDS-staged phi tile with a radius-2
 //              halo marched along z, masked normals -> curvature -> F,
@@ -59,6 +60,16 @@ __device__ __forceinline__ bool is_interior(const KParams &k, int x, int y, int 
 
 // --------------------------------------------------------------------------
 // K1: phi = sum of the seven arriving phase populations (phaseField.cuh:31-52).
+//
+// The phase populations are stored in moment form, four planes of g:
+// g_phi = g[0..N), o = (g[N..2N), g[2N..3N), g[3N..4N)), one value per source
+// cell s, with population k leaving s equal to
+//     w_k (g_phi(s) + c_k . o(s)),  w_0 = 1/4, w_1..6 = 1/8.
+// Every population the collision emits has this form (phase_moments), and the
+// boundary kernels write the one population they set as g_phi with o = 0.
+// The arriving sum below forms each population with the same two operations
+// the collision used (an add, then a power-of-two scale), so phi is the sum of
+// the same seven FP32 values, in the same order, as with population storage.
 __device__ __forceinline__ void phase_xyz(const KParams &k, const float *__restrict__ g, float *__restrict__ phi,
                                           int x, int y, int z) {
     const uint32_t c = cell_id(k, x, y, z);
@@ -81,13 +92,20 @@ __device__ __forceinline__ void phase_xyz(const KParams &k, const float *__restr
     }
     const size_t N = k.N;
     const int ci = int(c);
-    float s = g[c];
-    s += g[N + uint32_t(ci - 1 + sxp)];
-    s += g[2 * N + uint32_t(ci + 1 + sxm)];
-    s += g[3 * N + uint32_t(ci - k.nx + syp)];
-    s += g[4 * N + uint32_t(ci + k.nx + sym)];
-    s += g[5 * N + uint32_t(ci - int(k.sxy) + szp)];
-    s += g[6 * N + uint32_t(ci + int(k.sxy) + szm)];
+    const float *__restrict__ gp = g;
+    const float *__restrict__ ox = g + N;
+    const float *__restrict__ oy = g + 2 * N;
+    const float *__restrict__ oz = g + 3 * N;
+    const uint32_t s1 = uint32_t(ci - 1 + sxp), s2 = uint32_t(ci + 1 + sxm);
+    const uint32_t s3 = uint32_t(ci - k.nx + syp), s4 = uint32_t(ci + k.nx + sym);
+    const uint32_t s5 = uint32_t(ci - int(k.sxy) + szp), s6 = uint32_t(ci + int(k.sxy) + szm);
+    float s = weight7(0) * gp[c];
+    s += weight7(1) * (gp[s1] + ox[s1]);
+    s += weight7(2) * (gp[s2] - ox[s2]);
+    s += weight7(3) * (gp[s3] + oy[s3]);
+    s += weight7(4) * (gp[s4] - oy[s4]);
+    s += weight7(5) * (gp[s5] + oz[s5]);
+    s += weight7(6) * (gp[s6] - oz[s6]);
     phi[c] = s;
 }
 
@@ -103,14 +121,24 @@ __device__ __forceinline__ void phase_cell(const KParams &k, const float *__rest
 }
 
 #ifdef SSMD_STEP_KERNELS
-// Grid (ceil(nx / 64), ceil((ny - 2) / 4), nz - 2) of 64 x 4 blocks over the
-// interior rows: coordinates come from the launch, not from divisions.
+// 64 x 4 blocks over the interior rows, one tile per block: tiles are
+// numbered x-block fastest, then y-block, then plane z = 1..nz-2, and the
+// launch has exactly one block per tile.  Workgroups are dealt to the 8 XCDs
+// in turn (block b runs on XCD b % 8), so block b takes tile number
+// start(b % 8) + b / 8: each XCD works through one contiguous range of tiles
+// and finds the stencil neighbours of a tile (x +- 1 lines, y +- 1 rows,
+// planes z +- 1) in its own L2.  The mapping is a bijection for any count.
 // The ghost-column lanes (x = 0, nx-1) store back the phi they hold (boundary
 // data of the previous step) so that every phi line is written whole.
 __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi) {
-    const int x = int(blockIdx.x) * 64 + int(threadIdx.x);
-    const int y = 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
-    const int z = 1 + int(blockIdx.z);
+    const uint32_t tiles = gridDim.x, b = blockIdx.x, xcd = b & 7u;
+    const uint32_t per = tiles >> 3, extra = tiles & 7u;  // XCD j owns per + (j < extra) tiles
+    const uint32_t t = xcd * per + (xcd < extra ? xcd : extra) + (b >> 3);
+    const uint32_t bx = uint32_t(k.nx + 63) >> 6, by = uint32_t(k.ny - 2 + 3) >> 2;
+    const uint32_t row = t / bx;
+    const int x = int(t - row * bx) * 64 + int(threadIdx.x);
+    const int y = 1 + int(row % by) * 4 + int(threadIdx.y);
+    const int z = 1 + int(row / by);
     if (x > k.nx - 1 || y > k.ny - 2) return;
     if (x == 0 || x == k.nx - 1) {
         const uint32_t c = cell_id(k, x, y, z);
@@ -221,6 +249,13 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         const int sh = wrap_shift(k.wrap_x, k.nx);
         if (x == 1) xsp = sh;
         if (x == k.nx - 2) xsm = -sh;
+    }
+    // Ghost-column lanes pull only to keep the wave's loads uniform (their
+    // results are discarded): cancel the x step so they read their own
+    // column, on lines the interior lanes' c_x = 0 pulls fetch anyway.
+    if (ghost_x) {
+        xsp = 1;
+        xsm = -1;
     }
     long ysp = 0, ysm = 0;
     if (k.wrap_y) {
@@ -454,7 +489,7 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         });
         const float sharp = k.gamma * ph * (1.0f - ph);
         const uint32_t go = c * 4u;
-        phase_collide(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) {
+        phase_moments(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) {
             float *dst = reinterpret_cast<float *>(reinterpret_cast<char *>(gout + size_t(kk) * Ns) + go);
             __builtin_nontemporal_store(active ? v : 0.f, dst);
         });
@@ -696,9 +731,10 @@ __device__ __forceinline__ void inflow_face(const KParams &k, typename P::T *__r
                     }
                 }
             });
-            const int kk = AXIS == 1 ? 3 : 5;
+            // population +axis of the source bnd, w (ph (1 + 4 un)), in moment form
             const float un = AXIS == 1 ? uy : uz;
-            gout[size_t(kk) * k.N + bnd] = weight7(kk) * ph * (1.0f + 4.0f * un);
+            gout[bnd] = ph * (1.0f + 4.0f * un);
+            gout[size_t(AXIS + 1) * k.N + bnd] = 0.f;
         }
     }
 }
@@ -734,9 +770,10 @@ __device__ __forceinline__ void outflow_face(const KParams &k, typename P::T *__
                 }
             }
         });
-        const int kk = AXIS == 1 ? 4 : 6;
+        // population -axis of the source bnd, w (ph (1 - 4 un)), in moment form
         const float un = AXIS == 1 ? uy : uz;
-        gout[size_t(kk) * k.N + bnd] = weight7(kk) * ph * (1.0f - 4.0f * un);
+        gout[bnd] = ph * (1.0f - 4.0f * un);
+        gout[size_t(AXIS + 1) * k.N + bnd] = 0.f;
     }
 }
 
@@ -917,19 +954,16 @@ __global__ void __launch_bounds__(256) k_init(KParams k, typename P::T *f0, type
         f0[size_t(i) * k.N + c] = sv;
         f1[size_t(i) * k.N + c] = sv;
     });
-    sfor<0, 7>([&](auto K) {
-        constexpr int kk = K;
-        constexpr Dir d = kDir[kk];
-        const int ax = x + d.x, ay = y + d.y, az = z + d.z;
-        float v = 0.f;
-        if (ax >= 0 && ax < k.nx && ay >= 0 && ay < k.ny && az >= 0 && az < k.nz) {
-            float aph, aux, auy, auz;
-            init_macros(k, ax, ay, az, aph, aux, auy, auz);
-            v = weight7(kk) * aph * (1.0f + 4.0f * (float(d.x) * aux + float(d.y) * auy + float(d.z) * auz));
-        }
-        g0[size_t(kk) * k.N + c] = v;
-        g1[size_t(kk) * k.N + c] = v;
-    });
+    // Phase populations in moment form: all zero.  Every population an
+    // interior cell pulls at the first step is the t = 0 equilibrium of that
+    // interior cell (initialConditions.cu sets g to the local equilibrium),
+    // and interior cells start at phi = 0, u = 0, so each one is zero.
+    // Slots no interior cell pulls are never read.
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        g0[size_t(kk) * k.N + c] = 0.f;
+        g1[size_t(kk) * k.N + c] = 0.f;
+    }
 }
 
 // --------------------------------------------------------------------------

@@ -229,21 +229,21 @@ __device__ __forceinline__ void collide(const Cell &m, float omco, float fx, flo
     });
 }
 
-// D3Q7 post-collision phase population (D3Q7.cuh:514-538, lbm.cu:256-269):
-// g_k = w_k phi (1 + 4 c_k.u) + w_k gamma phi (1-phi) c_k.n
+// D3Q7 post-collision phase populations (D3Q7.cuh:514-538, lbm.cu:256-269):
+//     g_k = w_k phi (1 + 4 c_k.u) + w_k gamma phi (1-phi) c_k.n = w_k (phi + c_k.o),
+//     o = 4 phi u + gamma phi (1-phi) n.
+// All seven are fixed by (phi, o), which is what is stored: emit(0, phi),
+// emit(1 + a, o_a) for a = x, y, z.  The arriving sum rebuilds population
+// +a as w (phi + o_a) and -a as w (phi - o_a), the operations the reference's
+// per-population form performs on the same phi and o_a.
 template <class Emit>
-__device__ __forceinline__ void phase_collide(float phi, float sharp, float ux, float uy, float uz, float nx,
+__device__ __forceinline__ void phase_moments(float phi, float sharp, float ux, float uy, float uz, float nx,
                                               float ny, float nz, Emit &&emit) {
-    emit(0, weight7(0) * phi);
+    emit(0, phi);
     const float u[3] = {ux, uy, uz};
     const float n[3] = {nx, ny, nz};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float e = phi;
-        const float o = 4.0f * phi * u[a] + sharp * n[a];
-        emit(1 + 2 * a, weight7(1) * (e + o));
-        emit(2 + 2 * a, weight7(1) * (e - o));
-    }
+    for (int a = 0; a < 3; ++a) emit(1 + a, 4.0f * phi * u[a] + sharp * n[a]);
 }
 
 }  // namespace ssmd

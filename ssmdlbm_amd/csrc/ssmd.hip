@@ -160,8 +160,9 @@ int face_axis_host(int kind) {
 
 int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
     const KParams k = make_kparams(p);
-    const dim3 grid(unsigned(p.nx + 63) / 64, unsigned(p.ny - 2 + 3) / 4, unsigned(p.nz - 2));
-    hipLaunchKernelGGL(k_phase, grid, dim3(64, 4), 0, s, k, d.g, d.phi);
+    // one block per 64 x 4 tile of interior rows (the kernel derives the tile)
+    const unsigned tiles = (unsigned(p.nx + 63) / 64) * (unsigned(p.ny - 2 + 3) / 4) * unsigned(p.nz - 2);
+    hipLaunchKernelGGL(k_phase, dim3(tiles), dim3(64, 4), 0, s, k, d.g, d.phi);
     return herr(hipGetLastError());
 }
 
@@ -347,7 +348,7 @@ size_t ssmd_bytes(const ssmd_params *p, int32_t which) {
     const size_t n = size_t(p->nx) * p->ny * p->nz;
     if (which == 0) return n * 4;
     if (which == 1) return n * size_t(p->q) * pop_size(p->pop_format);
-    if (which == 2) return n * 7 * 4;
+    if (which == 2) return n * 4 * 4;
     return 0;
 }
 

@@ -41,7 +41,7 @@ class Simulation:
                 continue  # never allocated, as in the reference (main.cu:55-70)
             self.t[name] = z(n)
         pdt = _POP_DTYPE[int(self.p.pop_format)]
-        self._pops = {"fa": z(q * n, dtype=pdt), "fb": z(q * n, dtype=pdt), "ga": z(7 * n), "gb": z(7 * n)}
+        self._pops = {"fa": z(q * n, dtype=pdt), "fb": z(q * n, dtype=pdt), "ga": z(4 * n), "gb": z(4 * n)}
         self.t["step_counter"] = z(1, dtype=torch.int64)
         self.t["reduce"] = z(abi.REDUCE_DOUBLES, dtype=torch.float64)
         self.d = abi.Fields()
@@ -168,15 +168,30 @@ class Simulation:
         raise abi.SsmdError("population buffer bookkeeping lost")
 
     def pops_source(self, which: str = "f") -> np.ndarray:
-        """Populations the next step reads, source-indexed, float64 (Q, nz, ny, nx)."""
+        """Populations the next step reads, source-indexed, float64 (Q, nz, ny, nx).
+
+        g is stored as 4 moment planes (g_moments); its seven populations are
+        formed here with the FP32 operations the phase kernel applies."""
         self.synchronize()
-        q = int(self.p.q) if which == "f" else 7
+        if which == "g":
+            return phase_populations(self.g_moments()).astype(np.float64)
+        q = int(self.p.q)
         return self._current(which).double().cpu().numpy().reshape((q,) + self.p.shape)
 
+    def g_moments(self) -> np.ndarray:
+        """The stored phase state, float32 (4, nz, ny, nx): g_phi, o_x, o_y, o_z."""
+        self.synchronize()
+        return self._current("g").cpu().numpy().reshape((4,) + self.p.shape)
+
     def set_pops_source(self, which: str, values: np.ndarray) -> None:
-        """Overwrite both buffers of f or g (source layout) with `values`."""
+        """Overwrite both buffers of f or g (source layout) with `values`.
+
+        For g the seven populations of every cell must have the moment form
+        w_k (g_phi + c_k . o); anything else raises ValueError."""
         self.synchronize()
         keys = ("fa", "fb") if which == "f" else ("ga", "gb")
+        if which == "g":
+            values = phase_moments(np.asarray(values, dtype=np.float64).reshape((7,) + self.p.shape))
         src = torch.from_numpy(np.ascontiguousarray(values, dtype=np.float64).reshape(-1))
         for k in keys:
             self._pops[k].copy_(src.to(self._pops[k].dtype).to(self.device))
@@ -208,6 +223,34 @@ def _source_axis(n: int, c: int, wrap: int):
         s[(s == n - 1) & (c < 0)] = 1
     s[~valid] = -1
     return s
+
+
+W7 = (0.25, 0.125, 0.125, 0.125, 0.125, 0.125, 0.125)
+
+
+def phase_populations(mom: np.ndarray) -> np.ndarray:
+    """Seven D3Q7 populations (float32) from the 4 stored moment planes:
+    k = 0: w_0 g_phi; k = 1 + 2a / 2 + 2a: w (g_phi +/- o_a), in FP32."""
+    m = np.asarray(mom, dtype=np.float32)
+    gp = m[0]
+    out = [np.float32(W7[0]) * gp]
+    for a in range(3):
+        out.append(np.float32(W7[1]) * (gp + m[1 + a]))
+        out.append(np.float32(W7[2]) * (gp - m[1 + a]))
+    return np.stack(out)
+
+
+def phase_moments(pops: np.ndarray, rtol: float = 1e-6) -> np.ndarray:
+    """Inverse of phase_populations: (g_phi, o_x, o_y, o_z) from seven
+    populations of the form w_k (g_phi + c_k . o); ValueError otherwise."""
+    g = np.asarray(pops, dtype=np.float64)
+    gp = g[0] / W7[0]
+    mom = [gp] + [(g[1 + 2 * a] - g[2 + 2 * a]) / (2 * W7[1]) for a in range(3)]
+    back = phase_populations(np.stack(mom)).astype(np.float64)
+    scale = max(float(np.max(np.abs(g))), 1e-30)
+    if np.max(np.abs(back - g)) > rtol * scale + 1e-12:
+        raise ValueError("phase populations are not of the form w_k (g_phi + c_k . o)")
+    return np.stack(mom)
 
 
 def source_to_arrival(src: np.ndarray, params: abi.Params | None = None) -> np.ndarray:

@@ -86,7 +86,9 @@ def test_jet_case_constants():
     assert cases.algorithmic_bytes_per_cell(19, abi.POP_F32) == 216
     assert cases.algorithmic_bytes_per_cell(27, abi.POP_F32) == 280
     assert cases.algorithmic_bytes_per_cell(19, abi.POP_F16) == 140
-    assert cases.lattice_kernel_bytes_per_cell(27, abi.POP_F32) == 248
+    assert cases.lattice_kernel_bytes_per_cell(27, abi.POP_F32) == 236
+    assert cases.step_bytes_per_cell(27, abi.POP_F32) == 256
+    assert cases.step_bytes_per_cell(19, abi.POP_F16) == 116
 
 
 def test_validation_rejects_bad_parameters():
@@ -109,7 +111,7 @@ def test_validation_rejects_bad_parameters():
     assert lib.ssmd_bytes(p, 1) == 19 * 4 * p.cells
     b = p.copy()
     b.pop_format = abi.POP_F16
-    assert lib.ssmd_bytes(b, 1) == 19 * 2 * p.cells and lib.ssmd_bytes(b, 2) == 7 * 4 * p.cells
+    assert lib.ssmd_bytes(b, 1) == 19 * 2 * p.cells and lib.ssmd_bytes(b, 2) == 4 * 4 * p.cells
 
 
 def test_error_paths_without_device():

@@ -108,7 +108,10 @@ def test_init_state(gpu, q, which):
     for w in ("f", "g"):
         a = sim.pops_arrival(w)
         b = o.pops(w)
-        m = ~np.isnan(a)
+        # f: every slot with an in-domain source.  g is stored in moment form,
+        # which holds exactly the populations interior cells pull (the ghost
+        # arrivals of the t = 0 equilibrium are never read by any kernel).
+        m = ~np.isnan(a) if w == "f" else interior_arrivals(a)
         np.testing.assert_allclose(a[m], b[m], rtol=0, atol=1e-8)
 
 
