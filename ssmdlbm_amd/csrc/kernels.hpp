@@ -4,7 +4,7 @@
 //   scalar fields  float[N], idx = x + nx*(y + ny*z)
 //   f, f_next      Q planes of N pop elements (f32 / f16 / bf16 deviations)
 //   g, g_next      4 planes of N floats: the D3Q7 phase populations in
-//                  moment form (g_phi, o_x, o_y, o_z), see phase_xyz
+//                  moment form (g_phi, o_x, o_y, o_z), see phase_column
 // Populations are source-indexed ("pull"): slot [i][s] holds what leaves
 // cell s along c_i; the population arriving at cell a along c_i is slot
 // [i][a - c_i].  Every f element is read once and written once per step.
@@ -39,6 +39,10 @@ struct KParams {
     uint64_t seed;
     JetK water, oil;
     int wrap_x, wrap_y, wrap_z;  // SSMD_WRAP_*
+    // planes whose moments a boundary kernel reads on every step: bit 0 y = 1
+    // (water inlet), bit 1 y = ny-2 (outflow y), bit 2 z = 1 (oil inlet),
+    // bit 3 z = nz-2 (outflow z)
+    int bc_planes;
     int closure;                 // SSMD_CLOSURE_*
     float smag;                  // 18 Cs^2
     float strain_scale, grad_scale;
@@ -59,6 +63,30 @@ __device__ __forceinline__ bool is_interior(const KParams &k, int x, int y, int 
 }
 
 // --------------------------------------------------------------------------
+// Workgroups are dealt to the 8 XCDs in turn (block b runs on XCD b % 8), so
+// block b of a launch of `blocks` blocks takes work unit start(b % 8) + b / 8:
+// each XCD works through one contiguous range of units and finds the stencil
+// neighbours of a unit (x +- 1 lines, y +- 1 rows, planes z +- 1) in its own
+// L2.  A bijection of [0, blocks) for any count.
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t blocks) {
+    const uint32_t xcd = b & 7u, per = blocks >> 3, extra = blocks & 7u;  // XCD j owns per + (j < extra)
+    return xcd * per + (xcd < extra ? xcd : extra) + (b >> 3);
+}
+
+// Ghost-column cells (x = 0, nx-1) of interior rows store back the phi they
+// hold (boundary data of the previous step) so that every phi line a phase
+// kernel writes is written whole.
+__device__ __forceinline__ void phase_keep(float *__restrict__ phi, uint32_t c) {
+    float v = phi[c];
+    asm volatile("" : "+v"(v));  // an opaque value: the store is not elided
+    phi[c] = v;
+}
+
+// The single-box phase kernel: 64 x 4 tiles of interior rows, each marching
+// over kPhaseLZ planes; tiles numbered x-block fastest, then y-block, then
+// z-chunk, one block per tile: ceil(nx/64) ceil((ny-2)/4) ceil((nz-2)/kPhaseLZ).
+constexpr int kPhaseLZ = 4;
+
 // K1: phi = sum of the seven arriving phase populations (phaseField.cuh:31-52).
 //
 // The phase populations are stored in moment form, four planes of g:
@@ -70,11 +98,14 @@ __device__ __forceinline__ bool is_interior(const KParams &k, int x, int y, int 
 // The arriving sum below forms each population with the same two operations
 // the collision used (an add, then a power-of-two scale), so phi is the sum of
 // the same seven FP32 values, in the same order, as with population storage.
-__device__ __forceinline__ void phase_xyz(const KParams &k, const float *__restrict__ g, float *__restrict__ phi,
-                                          int x, int y, int z) {
-    const uint32_t c = cell_id(k, x, y, z);
-    // source slots across a periodic face are read from their periodic image
-    int sxp = 0, sxm = 0, syp = 0, sym = 0, szp = 0, szm = 0;
+//
+// One column (x, y), planes z0..z1-1: the +z and -z sources of a plane are
+// the own column's values of the planes below and above, carried in
+// registers from plane to plane (each plane loads only the one above).
+// Source slots across a periodic face are read from their periodic image.
+__device__ __forceinline__ void phase_column(const KParams &k, const float *__restrict__ g, float *__restrict__ phi,
+                                             int x, int y, int z0, int z1) {
+    int sxp = 0, sxm = 0, syp = 0, sym = 0;
     if (k.wrap_x) {
         const int sh = k.wrap_x == SSMD_WRAP_TRUE ? k.nx - 2 : k.nx - 3;
         if (x == 1) sxp = sh;
@@ -85,69 +116,53 @@ __device__ __forceinline__ void phase_xyz(const KParams &k, const float *__restr
         if (y == 1) syp = sh;
         if (y == k.ny - 2) sym = -sh;
     }
-    if (k.wrap_z) {
-        const int sh = (k.wrap_z == SSMD_WRAP_TRUE ? k.nz - 2 : k.nz - 3) * int(k.sxy);
-        if (z == 1) szp = sh;
-        if (z == k.nz - 2) szm = -sh;
-    }
+    const int zsh = k.wrap_z ? (k.wrap_z == SSMD_WRAP_TRUE ? k.nz - 2 : k.nz - 3) * int(k.sxy) : 0;
     const size_t N = k.N;
-    const int ci = int(c);
     const float *__restrict__ gp = g;
     const float *__restrict__ ox = g + N;
     const float *__restrict__ oy = g + 2 * N;
     const float *__restrict__ oz = g + 3 * N;
-    const uint32_t s1 = uint32_t(ci - 1 + sxp), s2 = uint32_t(ci + 1 + sxm);
-    const uint32_t s3 = uint32_t(ci - k.nx + syp), s4 = uint32_t(ci + k.nx + sym);
-    const uint32_t s5 = uint32_t(ci - int(k.sxy) + szp), s6 = uint32_t(ci + int(k.sxy) + szm);
-    float s = weight7(0) * gp[c];
-    s += weight7(1) * (gp[s1] + ox[s1]);
-    s += weight7(2) * (gp[s2] - ox[s2]);
-    s += weight7(3) * (gp[s3] + oy[s3]);
-    s += weight7(4) * (gp[s4] - oy[s4]);
-    s += weight7(5) * (gp[s5] + oz[s5]);
-    s += weight7(6) * (gp[s6] - oz[s6]);
-    phi[c] = s;
-}
-
-__device__ __forceinline__ void phase_cell(const KParams &k, const float *__restrict__ g, float *__restrict__ phi,
-                                           uint32_t c) {
-    if (c >= k.N) return;
-    const int x = int(c % uint32_t(k.nx));
-    const uint32_t r = c / uint32_t(k.nx);
-    const int y = int(r % uint32_t(k.ny));
-    const int z = int(r / uint32_t(k.ny));
-    if (!is_interior(k, x, y, z)) return;
-    phase_xyz(k, g, phi, x, y, z);
+    const int sxy = int(k.sxy);
+    int ci = int(cell_id(k, x, y, z0));
+    const uint32_t sb = uint32_t(ci - sxy + (z0 == 1 ? zsh : 0));
+    float bg = gp[sb], bz = oz[sb];  // source of the +z population (plane below)
+    float cg = gp[ci], cz = oz[ci];  // own column, this plane
+    for (int z = z0; z < z1; ++z, ci += sxy) {
+        const uint32_t sa = uint32_t(ci + sxy - (z == k.nz - 2 ? zsh : 0));
+        const float ag = gp[sa], az = oz[sa];  // source of the -z population (plane above)
+        const uint32_t s1 = uint32_t(ci - 1 + sxp), s2 = uint32_t(ci + 1 + sxm);
+        const uint32_t s3 = uint32_t(ci - k.nx + syp), s4 = uint32_t(ci + k.nx + sym);
+        float s = weight7(0) * cg;
+        s += weight7(1) * (gp[s1] + ox[s1]);
+        s += weight7(2) * (gp[s2] - ox[s2]);
+        s += weight7(3) * (gp[s3] + oy[s3]);
+        s += weight7(4) * (gp[s4] - oy[s4]);
+        s += weight7(5) * (bg + bz);
+        s += weight7(6) * (ag - az);
+        phi[ci] = s;
+        bg = cg;
+        bz = cz;
+        cg = ag;
+        cz = az;
+    }
 }
 
 #ifdef SSMD_STEP_KERNELS
-// 64 x 4 blocks over the interior rows, one tile per block: tiles are
-// numbered x-block fastest, then y-block, then plane z = 1..nz-2, and the
-// launch has exactly one block per tile.  Workgroups are dealt to the 8 XCDs
-// in turn (block b runs on XCD b % 8), so block b takes tile number
-// start(b % 8) + b / 8: each XCD works through one contiguous range of tiles
-// and finds the stencil neighbours of a tile (x +- 1 lines, y +- 1 rows,
-// planes z +- 1) in its own L2.  The mapping is a bijection for any count.
-// The ghost-column lanes (x = 0, nx-1) store back the phi they hold (boundary
-// data of the previous step) so that every phi line is written whole.
+// One block per phase tile, dealt to the XCDs in contiguous ranges.
 __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi) {
-    const uint32_t tiles = gridDim.x, b = blockIdx.x, xcd = b & 7u;
-    const uint32_t per = tiles >> 3, extra = tiles & 7u;  // XCD j owns per + (j < extra) tiles
-    const uint32_t t = xcd * per + (xcd < extra ? xcd : extra) + (b >> 3);
+    const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t bx = uint32_t(k.nx + 63) >> 6, by = uint32_t(k.ny - 2 + 3) >> 2;
     const uint32_t row = t / bx;
     const int x = int(t - row * bx) * 64 + int(threadIdx.x);
     const int y = 1 + int(row % by) * 4 + int(threadIdx.y);
-    const int z = 1 + int(row / by);
+    const int z0 = 1 + int(row / by) * kPhaseLZ;
+    const int z1 = min(z0 + kPhaseLZ, k.nz - 1);
     if (x > k.nx - 1 || y > k.ny - 2) return;
     if (x == 0 || x == k.nx - 1) {
-        const uint32_t c = cell_id(k, x, y, z);
-        float v = phi[c];
-        asm volatile("" : "+v"(v));  // an opaque value: the store is not elided
-        phi[c] = v;
+        for (int z = z0; z < z1; ++z) phase_keep(phi, cell_id(k, x, y, z));
         return;
     }
-    phase_xyz(k, g, phi, x, y, z);
+    phase_column(k, g, phi, x, y, z0, z1);
 }
 #endif  // SSMD_STEP_KERNELS
 
@@ -494,9 +509,13 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             __builtin_nontemporal_store(active ? v : 0.f, dst);
         });
 
-        const bool face = fx_lo || fx_hi || fy_lo || fy_hi || fz_lo || fz_hi;
+        // rho, u, Pi on the planes the inlet / outflow kernels read them from
+        // (BoundaryConditions.cuh:90-92, 159-161, 224-226, 284-286); periodic
+        // faces read u only on macro steps, which write every moment
+        const bool bc_plane = ((k.bc_planes & 1) && fy_lo) || ((k.bc_planes & 2) && fy_hi) ||
+                              ((k.bc_planes & 4) && fz_lo) || ((k.bc_planes & 8) && fz_hi);
         if (!active) continue;  // ghost-column lanes write populations only
-        if (MACROS || face) {
+        if (MACROS || bc_plane) {
             mac.rho[c] = m.rho;
             mac.ux[c] = m.ux;
             mac.uy[c] = m.uy;
@@ -540,8 +559,8 @@ struct JetSlot {
     Macro mac;
     int tiles_x, tiles_y, lz;  // lattice tiling of this jet
     uint32_t tile_begin;       // first lattice tile id
-    uint32_t cell_block_begin; // first 256-cell block id (phase kernel)
-    uint32_t face_begin[3];    // first face block id for faces normal to x, y, z (64x4 blocks)
+    uint32_t cell_block_begin; // first block id of the phase kernel (k_phase_batch)
+    uint32_t face_begin[4];    // first face block id for faces normal to x, y, z and the x-y ring (64x4 blocks)
 };
 
 __device__ __forceinline__ int find_slot(const JetSlot *jets, int njets, uint32_t id, int which) {
@@ -599,11 +618,28 @@ __global__ void __launch_bounds__(256) k_derived(KParams k, const float *__restr
     }
 }
 
-// K1 for every jet: one 256-thread block per 256 cells of a jet.
+// K1 for every jet.  A block takes 256 consecutive cells of an x-y plane
+// and marches over kPhaseLZ planes (phase_column): jet j owns blocks
+// [cell_block_begin_j, cell_block_begin_{j+1}), numbered plane block fastest,
+// then z-chunk, ceil(nx ny / 256) ceil((nz-2) / kPhaseLZ) per jet.  Flat
+// 256-cell rows keep every block's accesses contiguous for any box width;
+// blocks run in launch order (an XCD-contiguous order measured 1.5x slower
+// on the ensemble's narrow boxes).
 __global__ void __launch_bounds__(256) k_phase_batch(const JetSlot *__restrict__ jets, int njets) {
-    const int j = find_slot(jets, njets, blockIdx.x, 1);
-    const JetSlot &js = jets[j];
-    phase_cell(js.k, js.g, js.phi, (blockIdx.x - js.cell_block_begin) * 256u + threadIdx.x);
+    const uint32_t b = blockIdx.x;
+    const JetSlot &js = jets[find_slot(jets, njets, b, 1)];
+    const KParams &k = js.k;
+    const uint32_t lb = b - js.cell_block_begin, pb = (k.sxy + 255u) / 256u;
+    const uint32_t zc = lb / pb, cxy = (lb - zc * pb) * 256u + threadIdx.x;
+    if (cxy >= k.sxy) return;
+    const int x = int(cxy % uint32_t(k.nx)), y = int(cxy / uint32_t(k.nx));
+    const int z0 = 1 + int(zc) * kPhaseLZ, z1 = min(z0 + kPhaseLZ, k.nz - 1);
+    if (y < 1 || y > k.ny - 2) return;
+    if (x == 0 || x == k.nx - 1) {
+        for (int z = z0; z < z1; ++z) phase_keep(js.phi, cell_id(k, x, y, z));
+        return;
+    }
+    phase_column(k, js.g, js.phi, x, y, z0, z1);
 }
 #endif  // SSMD_STEP_KERNELS
 
@@ -685,14 +721,32 @@ __device__ __forceinline__ uint64_t step_index(const uint64_t *counter, uint64_t
     return counter ? *counter : step;
 }
 __device__ __forceinline__ void bump(uint64_t *counter, int doit) {
-    if (doit && counter && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *counter += 1;
+    if (doit && counter && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 &&
+        threadIdx.y == 0)
+        *counter += 1;
 }
+
+// The inlet and outflow kernels spread one face cell over several threads:
+// layer `slot` (blockIdx.z) of the launch emits the slot-th population of
+// the Q that cross the face (component `SIGN` along `AXIS`), and layer 0 also
+// writes the cell's boundary moments and phase moments.
+template <int Q, int AXIS, int SIGN>
+constexpr int crossing_rank(int i) {
+    int r = 0;
+    for (int j = 0; j < i; ++j) r += (AXIS == 1 ? kDir[j].y : kDir[j].z) == SIGN;
+    return r;
+}
+template <int Q>
+constexpr int crossing_count() {
+    return crossing_rank<Q, 2, 1>(Q);  // the same on every axis and side: Q19 5, Q27 9
+}
+static_assert(crossing_count<19>() == 5 && crossing_count<27>() == 9, "populations crossing a face");
 
 // applyWaterInflow (axis 1, face y=0) / applyOilInflow (axis 2, face z=0).
 template <int Q, class P, int AXIS>
 __device__ __forceinline__ void inflow_face(const KParams &k, typename P::T *__restrict__ fout, float *__restrict__ gout,
                                             float *__restrict__ phi, const Macro &mac, const uint64_t *counter,
-                                            uint64_t step, int a, int b) {
+                                            uint64_t step, int a, int b, int slot) {
     const JetK &j = AXIS == 1 ? k.water : k.oil;
     const int nb = AXIS == 1 ? k.nz : k.ny;
     if (a < k.nx && b < nb) {
@@ -709,16 +763,23 @@ __device__ __forceinline__ void inflow_face(const KParams &k, typename P::T *__r
             u3[AXIS] += j.u;
             const float ux = u3[0], uy = u3[1], uz = u3[2];
             const float rho = 1.0f, ph = j.phi;
-            mac.rho[bnd] = rho;
-            phi[bnd] = ph;
-            mac.ux[bnd] = ux;
-            mac.uy[bnd] = uy;
-            mac.uz[bnd] = uz;
+            if (slot == 0) {
+                mac.rho[bnd] = rho;
+                phi[bnd] = ph;
+                mac.ux[bnd] = ux;
+                mac.uy[bnd] = uy;
+                mac.uz[bnd] = uz;
+                // population +axis of the source bnd, w (ph (1 + 4 un)), in moment form
+                const float un = AXIS == 1 ? uy : uz;
+                gout[bnd] = ph * (1.0f + 4.0f * un);
+                gout[size_t(AXIS + 1) * k.N + bnd] = 0.f;
+            }
             const float omco = AXIS == 1 ? k.omco_water : k.omco_oil;
             sfor<0, Q>([&](auto I) {
                 constexpr int i = I;
                 constexpr Dir c = kDir[i];
                 if constexpr ((AXIS == 1 ? c.y : c.z) == 1) {
+                    if (slot != crossing_rank<Q, AXIS, 1>(i)) return;
                     const int xx = x + c.x, yy = y + c.y, zz = z + c.z;
                     bool ok;
                     if constexpr (AXIS == 2)
@@ -731,10 +792,6 @@ __device__ __forceinline__ void inflow_face(const KParams &k, typename P::T *__r
                     }
                 }
             });
-            // population +axis of the source bnd, w (ph (1 + 4 un)), in moment form
-            const float un = AXIS == 1 ? uy : uz;
-            gout[bnd] = ph * (1.0f + 4.0f * un);
-            gout[size_t(AXIS + 1) * k.N + bnd] = 0.f;
         }
     }
 }
@@ -743,22 +800,29 @@ __device__ __forceinline__ void inflow_face(const KParams &k, typename P::T *__r
 template <int Q, class P, int AXIS>
 __device__ __forceinline__ void outflow_face(const KParams &k, typename P::T *__restrict__ fout,
                                              float *__restrict__ gout, float *__restrict__ phi, const Macro &mac,
-                                             int a, int b) {
+                                             int a, int b, int slot) {
     const int nb = AXIS == 1 ? k.nz : k.ny;
     if (a >= 1 && a <= k.nx - 2 && b >= 1 && b <= nb - 2) {
         const int x = a, y = AXIS == 1 ? k.ny - 1 : b, z = AXIS == 1 ? b : k.nz - 1;
         const uint32_t bnd = cell_id(k, x, y, z);
         const uint32_t in = AXIS == 1 ? cell_id(k, x, k.ny - 2, z) : cell_id(k, x, y, k.nz - 2);
         const float rho = mac.rho[in], ph = phi[in], ux = mac.ux[in], uy = mac.uy[in], uz = mac.uz[in];
-        mac.rho[bnd] = rho;
-        phi[bnd] = ph;
-        mac.ux[bnd] = ux;
-        mac.uy[bnd] = uy;
-        mac.uz[bnd] = uz;
+        if (slot == 0) {
+            mac.rho[bnd] = rho;
+            phi[bnd] = ph;
+            mac.ux[bnd] = ux;
+            mac.uy[bnd] = uy;
+            mac.uz[bnd] = uz;
+            // population -axis of the source bnd, w (ph (1 - 4 un)), in moment form
+            const float un = AXIS == 1 ? uy : uz;
+            gout[bnd] = ph * (1.0f - 4.0f * un);
+            gout[size_t(AXIS + 1) * k.N + bnd] = 0.f;
+        }
         sfor<0, Q>([&](auto I) {
             constexpr int i = I;
             constexpr Dir c = kDir[i];
             if constexpr ((AXIS == 1 ? c.y : c.z) == -1) {
+                if (slot != crossing_rank<Q, AXIS, -1>(i)) return;
                 const int xx = x + c.x;
                 const int yy = AXIS == 1 ? k.ny - 2 : y + c.y;
                 const int zz = AXIS == 1 ? z + c.z : k.nz - 2;
@@ -770,10 +834,6 @@ __device__ __forceinline__ void outflow_face(const KParams &k, typename P::T *__
                 }
             }
         });
-        // population -axis of the source bnd, w (ph (1 - 4 un)), in moment form
-        const float un = AXIS == 1 ? uy : uz;
-        gout[bnd] = ph * (1.0f - 4.0f * un);
-        gout[size_t(AXIS + 1) * k.N + bnd] = 0.f;
     }
 }
 
@@ -811,6 +871,44 @@ __device__ __forceinline__ void periodic_face(const KParams &k, float *__restric
     }
 }
 
+// periodicX then periodicY (the y mirror), as one pass over the ghost ring
+// of the x and y faces.  Position a < ny is the x-ghost pair of row y = a
+// (all rows, ghost rows included); a >= ny is the y-ghost pair of column
+// x = a - ny + 1 (interior columns).  Run in that order, the two copies leave
+// every ghost cell of the ring equal to one cell of the interior columns and
+// rows: its own row/column wrapped on the ghost axis -- for the four corner
+// columns wrapped on both axes, since periodicY copies the row periodicX has
+// just filled.  Every source is read here and written by no thread of this
+// pass, so all cells are written in one pass with the sequential result.
+__device__ __forceinline__ void periodic_xy(const KParams &k, float *__restrict__ phi, const Macro &mac, int a, int b,
+                                            bool with_u) {
+    if (a >= k.ny + k.nx - 2 || b >= k.nz) return;
+    uint32_t gL, gR, sL, sR;
+    if (a < k.ny) {
+        const int ys = a == 0 ? k.ny - 2 : (a == k.ny - 1 ? 1 : a);  // the row periodicY leaves
+        gL = cell_id(k, 0, a, b);
+        gR = cell_id(k, k.nx - 1, a, b);
+        sL = cell_id(k, k.nx - 2, ys, b);
+        sR = cell_id(k, 1, ys, b);
+    } else {
+        const int x = a - k.ny + 1;
+        gL = cell_id(k, x, 0, b);
+        gR = cell_id(k, x, k.ny - 1, b);
+        sL = cell_id(k, x, k.ny - 2, b);
+        sR = cell_id(k, x, 1, b);
+    }
+    phi[gL] = phi[sL];
+    phi[gR] = phi[sR];
+    if (with_u) {  // u on the ghost layer is read by no kernel; kept current on macro steps
+        mac.ux[gL] = mac.ux[sL];
+        mac.ux[gR] = mac.ux[sR];
+        mac.uy[gL] = mac.uy[sL];
+        mac.uy[gR] = mac.uy[sR];
+        mac.uz[gL] = mac.uz[sL];
+        mac.uz[gR] = mac.uz[sR];
+    }
+}
+
 // Face operations, in the order a step applies them (main.cu:165-169, then
 // the new y and z periodic copies).
 enum FaceKind {
@@ -824,16 +922,32 @@ enum FaceKind {
     kPeriodicYTrue,
     kPeriodicZRef,
     kPeriodicZTrue,
+    kPeriodicXY,    // periodicX followed by periodicY, in one launch (periodic_xy)
 };
 
+// Block layers (blockIdx.z) of a face kind's launch: one per crossing
+// population for inlets and outflows, one otherwise.
+template <int Q>
+constexpr int face_layers(int kind) {
+    return kind == kInflowY || kind == kInflowZ || kind == kOutflowY || kind == kOutflowZ ? crossing_count<Q>() : 1;
+}
+
+// The face (block range) a kind runs over: 0, 1, 2 = normal to x, y, z;
+// 3 = the ghost ring of the x and y faces (periodic_xy).
 constexpr int face_axis(int kind) {
-    return kind == kInflowY || kind == kOutflowY || kind == kPeriodicYRef || kind == kPeriodicYTrue
+    return kind == kPeriodicXY ? 3
+           : kind == kInflowY || kind == kOutflowY || kind == kPeriodicYRef || kind == kPeriodicYTrue
                ? 1
                : (kind == kPeriodicXRef || kind == kPeriodicXTrue ? 0 : 2);
 }
 
 // Extent (na, nb) of a face normal to `axis`: the two other axes in order.
 __device__ __host__ __forceinline__ void face_extent(int axis, int nx, int ny, int nz, int &na, int &nb) {
+    if (axis == 3) {  // ring positions (periodic_xy) x planes
+        na = ny + nx - 2;
+        nb = nz;
+        return;
+    }
     na = axis == 0 ? ny : nx;
     nb = axis == 2 ? ny : nz;
 }
@@ -841,17 +955,18 @@ __device__ __host__ __forceinline__ void face_extent(int axis, int nx, int ny, i
 template <int Q, class P, int KIND>
 __device__ __forceinline__ void face_op(const KParams &k, typename P::T *fout, float *gout, float *phi,
                                         const Macro &mac, const uint64_t *counter, uint64_t step, int a, int b,
-                                        bool macros) {
-    if constexpr (KIND == kInflowY) inflow_face<Q, P, 1>(k, fout, gout, phi, mac, counter, step, a, b);
-    if constexpr (KIND == kInflowZ) inflow_face<Q, P, 2>(k, fout, gout, phi, mac, counter, step, a, b);
-    if constexpr (KIND == kOutflowY) outflow_face<Q, P, 1>(k, fout, gout, phi, mac, a, b);
-    if constexpr (KIND == kOutflowZ) outflow_face<Q, P, 2>(k, fout, gout, phi, mac, a, b);
+                                        bool macros, int slot) {
+    if constexpr (KIND == kInflowY) inflow_face<Q, P, 1>(k, fout, gout, phi, mac, counter, step, a, b, slot);
+    if constexpr (KIND == kInflowZ) inflow_face<Q, P, 2>(k, fout, gout, phi, mac, counter, step, a, b, slot);
+    if constexpr (KIND == kOutflowY) outflow_face<Q, P, 1>(k, fout, gout, phi, mac, a, b, slot);
+    if constexpr (KIND == kOutflowZ) outflow_face<Q, P, 2>(k, fout, gout, phi, mac, a, b, slot);
     if constexpr (KIND == kPeriodicXRef) periodic_face<0>(k, phi, mac, a, b, macros);
     if constexpr (KIND == kPeriodicXTrue) periodic_face<0>(k, phi, mac, a, b, macros);
     if constexpr (KIND == kPeriodicYRef) periodic_face<1>(k, phi, mac, a, b, macros);
     if constexpr (KIND == kPeriodicYTrue) periodic_face<1>(k, phi, mac, a, b, macros);
     if constexpr (KIND == kPeriodicZRef) periodic_face<2>(k, phi, mac, a, b, macros);
     if constexpr (KIND == kPeriodicZTrue) periodic_face<2>(k, phi, mac, a, b, macros);
+    if constexpr (KIND == kPeriodicXY) periodic_xy(k, phi, mac, a, b, macros);
 }
 
 // flags: bit 0 = advance the device step counter (last face kernel of the
@@ -863,7 +978,7 @@ __global__ void __launch_bounds__(256) k_face(KParams k, typename P::T *fout, fl
                                               uint64_t *counter, uint64_t step, int flags) {
     const int a = blockIdx.x * 64 + threadIdx.x;
     const int b = blockIdx.y * 4 + threadIdx.y;
-    face_op<Q, P, KIND>(k, fout, gout, phi, mac, counter, step, a, b, (flags & 2) != 0);
+    face_op<Q, P, KIND>(k, fout, gout, phi, mac, counter, step, a, b, (flags & 2) != 0, int(blockIdx.z));
     if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, flags & 1);
 }
 
@@ -882,7 +997,7 @@ __global__ void __launch_bounds__(256) k_face_batch(const JetSlot *__restrict__ 
     const int b = int(lb / uint32_t(bxs)) * 4 + threadIdx.y;
     using PT = typename P::T;
     face_op<Q, P, KIND>(js.k, static_cast<PT *>(js.f_next), js.g_next, js.phi, js.mac, counter, step, a, b,
-                        (flags & 2) != 0);
+                        (flags & 2) != 0, int(blockIdx.z));
     if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, flags & 1);
 }
 

@@ -56,8 +56,16 @@ struct PopF32 {
 struct PopF16 {
     using T = uint16_t;
     static __device__ __forceinline__ float load(T v) { return float(__builtin_bit_cast(_Float16, v)); }
-    // IEEE conversion in the default round-to-nearest-even mode.
-    static __device__ __forceinline__ T store(float v) { return __builtin_bit_cast(uint16_t, _Float16(v)); }
+    // IEEE conversion of the FP32 value in the default round-to-nearest-even
+    // mode (the reference's __float2half of its float result, utils.cuh:77-83).
+    // The value is made opaque first: otherwise the compiler folds the last
+    // FMA that produced it into a mixed-precision FMA with an f16 result
+    // (v_fma_mixlo_f16), which rounds once from the exact product-sum and so
+    // differs from rounding the FP32 result in rare ties.
+    static __device__ __forceinline__ T store(float v) {
+        asm volatile("" : "+v"(v));
+        return __builtin_bit_cast(uint16_t, _Float16(v));
+    }
 };
 struct PopBF16 {
     using T = uint16_t;

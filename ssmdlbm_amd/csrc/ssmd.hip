@@ -66,6 +66,7 @@ KParams make_kparams(const ssmd_params &p) {
     k.wrap_x = p.wrap_x;
     k.wrap_y = p.wrap_y;
     k.wrap_z = p.wrap_z;
+    k.bc_planes = (p.water.enabled ? 1 : 0) | (p.outflow_y ? 2 : 0) | (p.oil.enabled ? 4 : 0) | (p.outflow_z ? 8 : 0);
     k.closure = p.closure;
     k.smag = float(18.0 * p.cs * p.cs);
     k.strain_scale = float(p.strain_scale);
@@ -126,8 +127,14 @@ int face_kinds(const ssmd_params &p, int *kinds) {
     if (p.oil.enabled) kinds[n++] = kInflowZ;
     if (p.outflow_y) kinds[n++] = kOutflowY;
     if (p.outflow_z) kinds[n++] = kOutflowZ;
-    if (p.wrap_x) kinds[n++] = p.wrap_x == SSMD_WRAP_TRUE ? kPeriodicXTrue : kPeriodicXRef;
-    if (p.wrap_y) kinds[n++] = p.wrap_y == SSMD_WRAP_TRUE ? kPeriodicYTrue : kPeriodicYRef;
+    // the two wrap modes differ only in the population copies, which the pull
+    // addresses carry; the ghost phi/u wrap of x then y is one pass
+    if (p.wrap_x && p.wrap_y) {
+        kinds[n++] = kPeriodicXY;
+    } else {
+        if (p.wrap_x) kinds[n++] = p.wrap_x == SSMD_WRAP_TRUE ? kPeriodicXTrue : kPeriodicXRef;
+        if (p.wrap_y) kinds[n++] = p.wrap_y == SSMD_WRAP_TRUE ? kPeriodicYTrue : kPeriodicYRef;
+    }
     if (p.wrap_z) kinds[n++] = p.wrap_z == SSMD_WRAP_TRUE ? kPeriodicZTrue : kPeriodicZRef;
     return n;
 }
@@ -148,20 +155,18 @@ void face_dispatch(int kind, Fn &&fn) {
         case kPeriodicYTrue: fn.template operator()<kPeriodicYTrue>(); break;
         case kPeriodicZRef: fn.template operator()<kPeriodicZRef>(); break;
         case kPeriodicZTrue: fn.template operator()<kPeriodicZTrue>(); break;
+        case kPeriodicXY: fn.template operator()<kPeriodicXY>(); break;
         default: break;
     }
 }
 
-int face_axis_host(int kind) {
-    return (kind == kInflowY || kind == kOutflowY || kind == kPeriodicYRef || kind == kPeriodicYTrue)
-               ? 1
-               : ((kind == kPeriodicXRef || kind == kPeriodicXTrue) ? 0 : 2);
-}
+int face_axis_host(int kind) { return face_axis(kind); }
 
 int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
     const KParams k = make_kparams(p);
-    // one block per 64 x 4 tile of interior rows (the kernel derives the tile)
-    const unsigned tiles = (unsigned(p.nx + 63) / 64) * (unsigned(p.ny - 2 + 3) / 4) * unsigned(p.nz - 2);
+    // one block per 64 x 4 tile of interior rows and kPhaseLZ planes (the kernel derives the tile)
+    const unsigned tiles = (unsigned(p.nx + 63) / 64) * (unsigned(p.ny - 2 + 3) / 4) *
+                           unsigned((p.nz - 2 + kPhaseLZ - 1) / kPhaseLZ);
     hipLaunchKernelGGL(k_phase, dim3(tiles), dim3(64, 4), 0, s, k, d.g, d.phi);
     return herr(hipGetLastError());
 }
@@ -205,7 +210,7 @@ int launch_boundaries(const ssmd_params &p, const ssmd_fields &d, uint64_t *coun
         for (int i = 0; i < nk; ++i) {
             int na, nb;
             face_extent(face_axis_host(kinds[i]), k.nx, k.ny, k.nz, na, nb);
-            const dim3 grid((na + 63) / 64, (nb + 3) / 4);
+            const dim3 grid((na + 63) / 64, (nb + 3) / 4, face_layers<Q>(kinds[i]));
             face_dispatch(kinds[i], [&]<int KIND>() {
                 const int fl = int(i == last) | ((sflags & SSMD_STEP_MACROS) ? 2 : 0);
                 hipLaunchKernelGGL((k_face<Q, P, KIND>), grid, dim3(64, 4), 0, s, k, fo, d.g_next, d.phi, m, counter,
@@ -701,7 +706,7 @@ struct ssmd_ensemble {
     int kinds[8] = {};
     int nk = 0, last = -1;
     JetSlot *table[2] = {nullptr, nullptr};
-    uint32_t total_tiles = 0, cell_blocks = 0, face_blocks[3] = {0, 0, 0};
+    uint32_t total_tiles = 0, phase_blocks = 0, face_blocks[4] = {0, 0, 0, 0};
     int parity = 0;
     uint64_t *counter = nullptr;
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -723,7 +728,7 @@ void ensemble_drop_graphs(ssmd_ensemble *e) {
 int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total_tiles) {
     SSMD_HIP(hipDeviceSynchronize());  // no launch may still read the tables being replaced
     std::vector<JetSlot> h[2];
-    uint32_t tiles = 0, cells = 0, faces[3] = {0, 0, 0};
+    uint32_t tiles = 0, pblocks = 0, faces[4] = {0, 0, 0, 0};
     const TileShape ts = kTiles[tile];
     for (int j = 0; j < e->n; ++j) {
         const ssmd_params &pp = e->p[j];
@@ -738,9 +743,9 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
         const int tz = (int(pp.nz) - 2 + js.lz - 1) / js.lz;
         js.tile_begin = tiles;
         tiles += uint32_t(js.tiles_x) * uint32_t(js.tiles_y) * uint32_t(tz);
-        js.cell_block_begin = cells;
-        cells += (js.k.N + 255u) / 256u;
-        for (int ax = 0; ax < 3; ++ax) {
+        js.cell_block_begin = pblocks;
+        pblocks += ((js.k.sxy + 255u) / 256u) * uint32_t((pp.nz - 2 + kPhaseLZ - 1) / kPhaseLZ);
+        for (int ax = 0; ax < 4; ++ax) {
             int na, nb;
             face_extent(ax, js.k.nx, js.k.ny, js.k.nz, na, nb);
             js.face_begin[ax] = faces[ax];
@@ -758,15 +763,15 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
     for (int par = 0; par < 2; ++par)
         SSMD_HIP(hipMemcpy(dst[par], h[par].data(), sizeof(JetSlot) * size_t(e->n), hipMemcpyHostToDevice));
     *total_tiles = tiles;
-    e->cell_blocks = cells;
-    for (int ax = 0; ax < 3; ++ax) e->face_blocks[ax] = faces[ax];
+    e->phase_blocks = pblocks;
+    for (int ax = 0; ax < 4; ++ax) e->face_blocks[ax] = faces[ax];
     return SSMD_OK;
 }
 
 int ensemble_launch_step(ssmd_ensemble *e, const JetSlot *tab, int tile, uint32_t total_tiles, uint64_t *counter,
                          uint64_t step, uint32_t flags, hipStream_t s) {
     const ssmd_params &p0 = e->p[0];
-    hipLaunchKernelGGL(k_phase_batch, dim3(e->cell_blocks), dim3(256), 0, s, tab, e->n);
+    hipLaunchKernelGGL(k_phase_batch, dim3(e->phase_blocks), dim3(256), 0, s, tab, e->n);
     SSMD_HIP(hipGetLastError());
     const bool macros = (flags & SSMD_STEP_MACROS) != 0;
     if (macros && tile != 0) return SSMD_EINVAL;  // tables are laid out for the tile in use
@@ -778,7 +783,8 @@ int ensemble_launch_step(ssmd_ensemble *e, const JetSlot *tab, int tile, uint32_
             const int ax = face_axis_host(e->kinds[i]);
             face_dispatch(e->kinds[i], [&]<int KIND>() {
                 const int fl = int(i == e->last) | (macros ? 2 : 0);
-                hipLaunchKernelGGL((k_face_batch<Q, P, KIND>), dim3(e->face_blocks[ax]), dim3(64, 4), 0, s, tab, e->n,
+                hipLaunchKernelGGL((k_face_batch<Q, P, KIND>), dim3(e->face_blocks[ax], 1, face_layers<Q>(KIND)),
+                                   dim3(64, 4), 0, s, tab, e->n,
                                    counter, step, fl);
             });
         }
@@ -901,7 +907,7 @@ int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *be
     int bt = e->tile;
     const ssmd_params &p0 = e->p[0];
     if (rc == 0) {
-        hipLaunchKernelGGL(k_phase_batch, dim3(e->cell_blocks), dim3(256), 0, s, e->table[e->parity], e->n);
+        hipLaunchKernelGGL(k_phase_batch, dim3(e->phase_blocks), dim3(256), 0, s, e->table[e->parity], e->n);
         rc = herr(hipGetLastError());
     }
     for (int t = 0; t < kNumTiles && rc == 0; ++t) {

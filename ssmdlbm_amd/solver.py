@@ -22,7 +22,10 @@ _POP_DTYPE = {abi.POP_F32: torch.float32, abi.POP_F16: torch.float16, abi.POP_BF
 class Simulation:
     """One field set on one GPU (reference: the global LBMFields + FieldAllocate)."""
 
-    def __init__(self, params: abi.Params, device: str | torch.device = "cuda:0"):
+    def __init__(self, params: abi.Params, device: str | torch.device = "cuda:0",
+                 stream: torch.cuda.Stream | None = None):
+        """`stream`: run on this (non-null) stream instead of one of our own;
+        simulations sharing a stream execute strictly one after another."""
         self.lib = abi.load()
         abi.check(self.lib.ssmd_validate(C.byref(params)), "ssmd_validate")
         if not torch.cuda.is_available():
@@ -31,7 +34,7 @@ class Simulation:
         self.device = torch.device(device)
         torch.cuda.set_device(self.device)
         # a stream of our own (default priority): graph capture cannot use the null stream
-        self.stream = torch.cuda.Stream(self.device)
+        self.stream = stream if stream is not None else torch.cuda.Stream(self.device)
         n = self.p.cells
         q = int(self.p.q)
         z = lambda *shape, dtype=torch.float32: torch.zeros(*shape, dtype=dtype, device=self.device)

@@ -202,3 +202,38 @@ def test_derived_statistics_closed_forms(gpu):
         np.testing.assert_allclose(f[inner], val, rtol=1e-4)
         assert not f[~inner].any(), name
     np.testing.assert_allclose(ds.field("vort_mag")[inner], s * np.sqrt(a * a + b * b + c * c), rtol=1e-4)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_f16_store_is_rne_of_the_fp32_result(gpu, q):
+    """FP16 population storage rounds each post-collision FP32 value to
+    nearest-even, as the reference's __float2half of its float result
+    (utils.cuh:77-83): from one f16-representable state, the f16 step's
+    stored populations equal RNE(the FP32 step's), bit for bit, at every slot
+    (collision and boundary kernels alike)."""
+    base = cases.JET_D3Q19_SMALL.with_(nx=32, ny=32, nz=48, diameter=8, q=q, perturb=0.01).params()
+    p16 = base.copy()
+    p16.pop_format = abi.POP_F16
+    s16 = Simulation(p16)
+    s16.init()
+    s16.step(40)
+    s32 = Simulation(base)
+    s32.init()
+    s16.synchronize()
+    s32._current("f").copy_(s16._current("f").float())
+    s32._current("g").copy_(s16._current("g"))
+    for name, t in s16.t.items():
+        if t.dtype == torch.float32 and name in s32.t:
+            s32.t[name].copy_(t)
+    s32.step_index = s16.step_index
+    torch.cuda.synchronize()
+    s16.step(1)
+    s32.step(1)
+    s16.synchronize()
+    s32.synchronize()
+    got = s16._current("f").view(torch.int16).cpu().numpy()
+    want = s32._current("f").half().view(torch.int16).cpu().numpy()  # torch: IEEE RNE
+    bad = np.count_nonzero(got != want)
+    assert bad == 0, f"{bad} of {got.size} f16 slots differ from RNE of the FP32 step"
+    s16.close()
+    s32.close()

@@ -300,6 +300,38 @@ def test_reference_periodic_copy_is_not_conservative():
     assert abs(o.sums()[0] / m0 - 1) > 1e-6
 
 
+@pytest.mark.parametrize("wraps", [(1, 1, 1), (1, 1, 0), (0, 1, 1)])
+def test_oracle_ghost_wraps_are_sequential_plane_copies(wraps):
+    """The oracle's periodic kernels wrap phi and u on the ghost layer as the
+    reference does (BoundaryConditions.cuh:335-348): along x, then y, then z,
+    [0] <- [n-2] and [n-1] <- [1] on every row of the other two axes, ghost
+    rows included -- so a corner column ends up with the doubly wrapped value.
+    Restated here with numpy plane copies applied in that order."""
+    p = cases.closed_box(12, q=19)
+    p.nx, p.ny, p.nz = 13, 11, 9
+    p.wrap_x, p.wrap_y, p.wrap_z = (abi.WRAP_REFERENCE if w else 0 for w in wraps)
+    o = Oracle(p, fp64=True, threads=1)
+    o.init()
+    rng = np.random.default_rng(5)
+    names = ("phi", "ux", "uy", "uz")
+    want = {n: rng.standard_normal(p.shape) for n in names}
+    for n in names:
+        o.set_field(n, want[n])
+    o.k_boundaries(0)
+    for axis, on in zip((2, 1, 0), wraps):  # arrays are (z, y, x)
+        if not on:
+            continue
+        for v in want.values():
+            n = v.shape[axis]
+            v[(slice(None),) * axis + (0,)] = v[(slice(None),) * axis + (n - 2,)]
+            v[(slice(None),) * axis + (n - 1,)] = v[(slice(None),) * axis + (1,)]
+    for n in names:
+        np.testing.assert_array_equal(o.field(n), want[n], err_msg=n)
+    # the corner column really is the doubly wrapped interior value
+    if wraps[0] and wraps[1]:
+        assert o.field("phi")[3, 0, 0] == want["phi"][3, p.ny - 2, p.nx - 2]
+
+
 def _jet(**kw):
     args = dict(nx=22, ny=22, nz=30, diameter=8, perturb=0.0)
     args.update(kw)
