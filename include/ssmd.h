@@ -329,11 +329,16 @@ int ssmd_mlp_adam(float *params, const float *grad, float *m, float *v, uint32_t
                   float beta2, float eps, void *stream);
 /* Training samples of the LEARNED closure from a fine field set whose last
  * step wrote every moment (SSMD_STEP_MACROS), box-filtered by 2 onto the
- * coarse grid `coarse` (acoustic scaling: coarse interior cell i covers fine
- * cells 2i-1 and 2i per axis; requires 2 (n_c - 2) <= n_f - 2 on each axis).
+ * coarse grid `coarse` (coarse interior cell i covers fine cells 2i-1 and 2i
+ * per axis; requires 2 (n_c - 2) <= n_f - 2 on each axis).
  * work receives 17 coarse planes (ssmd_sgs_work_bytes): rho, Favre-filtered
  * ux uy uz, phi, the filtered strain rate S (xx yy zz xy xz yz, per coarse
- * step) and the subgrid stress T = <rho u u> - <rho> u~ u~ (same order).
+ * step) and the subgrid stress T = <rho u u> - <rho> u~ u~ (same order),
+ * all in coarse lattice units.  The two runs must describe one flow: with
+ * dx_c = 2 dx_f, their water viscosities give the time ratio rt = dt_c / dt_f
+ * = 4 nu_c / nu_f (2 for acoustic scaling, nu_f = 2 nu_c; 4 for diffusive
+ * scaling, nu_f = nu_c; 1 <= rt <= 16 or SSMD_EINVAL), and fine velocities,
+ * strain rates and stresses are scaled by rt / 2, rt and (rt / 2)^2.
  * Per coarse cell: feat = the 8 network inputs as the LEARNED step forms them
  * (coarse->strain_scale, grad_scale), target = 3 nu_t / coarse->tau_scale with
  * nu_t = -(dev T : S) / (2 rho S:S) the least-squares eddy viscosity, and

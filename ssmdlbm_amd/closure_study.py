@@ -1,16 +1,22 @@
 """Learned-closure study: training data from filtered fine-grid jets, training
 on the GPU, and the a-posteriori comparison of coarse runs.
 
-No counterpart in the reference (it has no subgrid model).  Geometry and
-scaling (acoustic: dx and dt halve on the fine grid, lattice velocities
-agree):
+No counterpart in the reference (it has no subgrid model).  The fine twin
+is the same physical jet at twice the resolution: 2n cells per axis,
+interface width 2W, diameter 2D, centre 2c - 1/2, so coarse interior cell i
+covers fine cells 2i-1, 2i (the box filter of ssmd_sgs_samples).  Time
+scaling:
 
-  coarse  nx x ny x nz, tau_c, sigma_c, inlet diameter D, centre c
-  fine    2n cells per axis, tau_f = 1/2 + 2 (tau_c - 1/2), sigma_f =
-          2 sigma_c, interface width 2W, diameter 2D, centre 2c - 1/2
+  diffusive (default)  dt_f = dt_c / 4: tau_f = tau_c, u_f = u_c / 2,
+          sigma_f = sigma_c / 2; one coarse step is four fine steps.  Every
+          physical parameter agrees, the interface mobility included (the
+          D3Q7 update relaxes fully, so its mobility is 1/8 in lattice units
+          on any grid, and dx^2 / dt is the same on both grids).
+  acoustic  dt_f = dt_c / 2: tau_f = 1/2 + 2 (tau_c - 1/2), u_f = u_c,
+          sigma_f = 2 sigma_c; one coarse step is two fine steps, but the fine
+          grid's physical interface mobility is half the coarse grid's.
 
-so coarse interior cell i covers fine cells 2i-1, 2i (the box filter of
-ssmd_sgs_samples) and one coarse step is two fine steps.
+In both, the sharpening coefficient of the fine grid is that of width 2W.
 
 Statistics, on the coarse grid for both (fine: Favre-filtered fields):
   centreline decay  time-averaged u_z on the four axis columns, per z;
@@ -22,6 +28,7 @@ fine one.
 Command line (one GPU):
   python -m ssmdlbm_amd.closure_study train --out DIR   fine runs -> network
   python -m ssmdlbm_amd.closure_study eval  --net DIR/closure --out DIR
+  python -m ssmdlbm_amd.closure_study noise --out DIR   seed-to-seed spread
 """
 from __future__ import annotations
 
@@ -58,6 +65,12 @@ class Geometry:
     diameter: int = 16
     perturb: float = 0.01
     cs: float = 0.1
+    scaling: str = "diffusive"  # or "acoustic" (module docstring)
+
+    @property
+    def fine_steps(self) -> int:
+        """Fine steps per coarse step."""
+        return 4 if self.scaling == "diffusive" else 2
 
 
 @dataclass
@@ -88,12 +101,19 @@ def coarse_params(g: Geometry, jet: Jet, closure: int = abi.CLOSURE_NONE,
 
 
 def fine_params(g: Geometry, jet: Jet) -> abi.Params:
-    """The same jet at twice the resolution (acoustic scaling)."""
+    """The same jet at twice the resolution (g.scaling: diffusive or acoustic)."""
     c = coarse_params(g, jet)
     p = c.copy()
     p.nx, p.ny, p.nz = 2 * g.nx, 2 * g.ny, 2 * g.nz
-    p.visc_water = p.visc_oil = viscosity_from_tau(0.5 + 2.0 * (g.tau - 0.5))
-    p.sigma = 2.0 * jet.sigma
+    if g.scaling == "diffusive":
+        p.visc_water = p.visc_oil = viscosity_from_tau(g.tau)
+        p.oil.u = 0.5 * jet.u
+        p.sigma = 0.5 * jet.sigma
+    elif g.scaling == "acoustic":
+        p.visc_water = p.visc_oil = viscosity_from_tau(0.5 + 2.0 * (g.tau - 0.5))
+        p.sigma = 2.0 * jet.sigma
+    else:
+        raise ValueError(f"unknown scaling {g.scaling!r}")
     # the same physical interface width: twice the cells (the sharpening
     # coefficient halves; the D3Q7 mobility is fixed in lattice units)
     p.gamma = gamma_from_tau_g(g.tau_g, 2.0 * g.width)
@@ -202,10 +222,11 @@ def compare(coarse: dict, filt: dict, z_range: tuple[int, int], kmax: int) -> di
 # -- fine runs ----------------------------------------------------------------
 def run_fine(g: Geometry, jet: Jet, coarse_steps: int, sample_every: int, stat_from: int,
              snapshot_steps: list[int], z_range, kmax, on_snapshot=None, tune: bool = True) -> dict:
-    """Fine run of `2 coarse_steps` steps; statistics of the filtered fields
-    every `sample_every` coarse steps from coarse step `stat_from`, and
-    on_snapshot(filter) at the listed coarse steps."""
+    """Fine run of `g.fine_steps * coarse_steps` steps; statistics of the
+    filtered fields every `sample_every` coarse steps from coarse step
+    `stat_from`, and on_snapshot(filter) at the listed coarse steps."""
     fp, cp = fine_params(g, jet), coarse_params(g, jet)
+    r = g.fine_steps
     sim = Simulation(fp)
     sim.init()
     if tune:
@@ -214,11 +235,11 @@ def run_fine(g: Geometry, jet: Jet, coarse_steps: int, sample_every: int, stat_f
     flt = Filter(fp, cp)
     stats = Stats(cp, z_range, kmax)
     marks = sorted(set([s for s in range(stat_from, coarse_steps + 1, sample_every)] + list(snapshot_steps)))
-    done = sim.step_index // 2
+    done = sim.step_index // r
     t0 = time.perf_counter()
     for m in marks:
-        if 2 * m - 1 > sim.step_index:
-            sim.graph_run(2 * m - 1 - sim.step_index)
+        if r * m - 1 > sim.step_index:
+            sim.graph_run(r * m - 1 - sim.step_index)
         sim.step(1, macros=True)
         flt(sim)
         if m >= stat_from and (m - stat_from) % sample_every == 0:
@@ -304,7 +325,7 @@ def fit_scales(g: Geometry, jets: list[Jet], probe_steps: int) -> Scales:
     fp, cp = fine_params(g, jet), coarse_params(g, jet, abi.CLOSURE_LEARNED, Scales(1.0, 1.0, 1.0))
     sim = Simulation(fp)
     sim.init()
-    sim.graph_run(2 * probe_steps - 1)
+    sim.graph_run(g.fine_steps * probe_steps - 1)
     sim.step(1, macros=True)
     flt = Filter(fp, cp)
     flt(sim)
@@ -325,7 +346,7 @@ def fit_scales(g: Geometry, jets: list[Jet], probe_steps: int) -> Scales:
 def train_main(args) -> int:
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
-    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q, tau=args.tau)
+    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q, tau=args.tau, scaling=args.scaling)
     rng = np.random.default_rng(args.seed)
     jets = [Jet(float(rng.uniform(*args.u_range)), float(rng.uniform(*args.sigma_range)), int(args.seed * 100 + j))
             for j in range(args.runs)]
@@ -369,6 +390,7 @@ def run_fine_scaled(g: Geometry, jet: Jet, snaps: list[int], scales: Scales, on_
     training scales (its coarse parameters carry them) and on_snap gets it."""
     fp = fine_params(g, jet)
     cp = coarse_params(g, jet, abi.CLOSURE_LEARNED, scales)
+    r = g.fine_steps
     sim = Simulation(fp)
     sim.init()
     sim.step(2)
@@ -376,8 +398,8 @@ def run_fine_scaled(g: Geometry, jet: Jet, snaps: list[int], scales: Scales, on_
     flt = Filter(fp, cp)
     t0 = time.perf_counter()
     for m in snaps:
-        if 2 * m - 1 > sim.step_index:
-            sim.graph_run(2 * m - 1 - sim.step_index)
+        if r * m - 1 > sim.step_index:
+            sim.graph_run(r * m - 1 - sim.step_index)
         sim.step(1, macros=True)
         flt(sim)
         on_snap(flt)
@@ -394,7 +416,7 @@ def run_fine_scaled(g: Geometry, jet: Jet, snaps: list[int], scales: Scales, on_
 def eval_main(args) -> int:
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
-    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q, tau=args.tau)
+    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q, tau=args.tau, scaling=args.scaling)
     jet = Jet(args.u, args.sigma, args.seed)
     net = Network.load(args.net)
     z_range = (1, args.nz // 2)
@@ -411,7 +433,7 @@ def eval_main(args) -> int:
     res["filtered_fine"] = run_fine(g, jet, args.steps, args.every, stat_from, [], z_range, kmax)
     res["errors"] = {k: compare(res[k], res["filtered_fine"], z_range, kmax)
                      for k in ("none", "smagorinsky", "learned")}
-    fine_cells = (2 * g.nx) * (2 * g.ny) * (2 * g.nz) * 2
+    fine_cells = (2 * g.nx) * (2 * g.ny) * (2 * g.nz) * g.fine_steps
     res["cell_update_ratio"] = fine_cells / (g.nx * g.ny * g.nz)
     (out / "eval_report.json").write_text(json.dumps(res))
     for k, v in res["errors"].items():
@@ -420,17 +442,46 @@ def eval_main(args) -> int:
     return 0
 
 
+def noise_main(args) -> int:
+    """The comparison's noise floor: the coarse no-closure jet of the
+    evaluation run with `--seeds` different inlet-noise seeds, and the errors
+    between every pair of them (same statistics, same window)."""
+    out = Path(args.out)
+    out.mkdir(parents=True, exist_ok=True)
+    g = Geometry(nx=args.nx, ny=args.ny, nz=args.nz, q=args.q, tau=args.tau, scaling=args.scaling)
+    z_range, kmax, stat_from = (1, args.nz // 2), args.kmax, args.steps // 2
+    runs = []
+    for k in range(args.seeds):
+        jet = Jet(args.u, args.sigma, args.seed + 1000 * k)
+        runs.append(run_coarse(g, jet, abi.CLOSURE_NONE, None, args.steps, args.every, stat_from, z_range, kmax))
+        log(f"coarse none, seed {jet.seed}: finite {runs[-1]['finite']} ({runs[-1]['seconds']:.1f} s)")
+    pairs = {f"{i}-{j}": compare(runs[i], runs[j], z_range, kmax)
+             for i in range(len(runs)) for j in range(i + 1, len(runs))}
+    keys = ("centreline_rel_l2", "spectrum_rel_l2", "interface_area_rel")
+    summary = {k: {"mean": float(np.mean([v[k] for v in pairs.values()])),
+                   "max": float(np.max([v[k] for v in pairs.values()]))} for k in keys}
+    rep = {"u": args.u, "sigma": args.sigma, "seeds": [args.seed + 1000 * k for k in range(args.seeds)],
+           "steps": args.steps, "stat_from": stat_from, "every": args.every, "geometry": asdict(g),
+           "pairs": pairs, "summary": summary}
+    (out / "noise_report.json").write_text(json.dumps(rep, indent=1))
+    for k, v in summary.items():
+        log(f"seed-to-seed {k}: mean {v['mean']:.4f}, max {v['max']:.4f}")
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
-    for name in ("train", "eval"):
+    for name in ("train", "eval", "noise"):
         s = sub.add_parser(name)
         s.add_argument("--out", required=True)
         s.add_argument("--nx", type=int, default=128)
         s.add_argument("--ny", type=int, default=128)
         s.add_argument("--nz", type=int, default=512)
         s.add_argument("--q", type=int, default=27)
-        s.add_argument("--steps", type=int, default=10000, help="coarse steps (fine runs take twice as many)")
+        s.add_argument("--steps", type=int, default=10000, help="coarse steps (the fine runs take 4 or 2 per step)")
+        s.add_argument("--scaling", choices=("diffusive", "acoustic"), default=Geometry.scaling,
+                       help="time scaling of the fine twin")
         s.add_argument("--seed", type=int, default=5)
         s.add_argument("--tau", type=float, default=Geometry.tau, help="coarse base relaxation time")
     t = sub.choices["train"]
@@ -448,11 +499,17 @@ def main(argv=None) -> int:
     e.add_argument("--sigma", type=float, default=0.02)
     e.add_argument("--every", type=int, default=50)
     e.add_argument("--kmax", type=int, default=32)
+    n = sub.choices["noise"]
+    n.add_argument("--u", type=float, default=0.05)
+    n.add_argument("--sigma", type=float, default=0.02)
+    n.add_argument("--every", type=int, default=50)
+    n.add_argument("--kmax", type=int, default=32)
+    n.add_argument("--seeds", type=int, default=4)
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("the closure study runs on the GPU", file=sys.stderr)
         return 1
-    return train_main(args) if args.cmd == "train" else eval_main(args)
+    return {"train": train_main, "eval": eval_main, "noise": noise_main}[args.cmd](args)
 
 
 if __name__ == "__main__":

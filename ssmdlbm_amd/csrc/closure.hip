@@ -475,6 +475,8 @@ struct Fine {
     int nx, ny, nz;
     const float *rho, *ux, *uy, *uz, *pxx, *pyy, *pzz, *pxy, *pxz, *pyz, *phi;
     float visc_water, visc_diff;
+    // fine -> coarse lattice units: rt = dt_c / dt_f (time), rv = rt / 2 (velocity)
+    float rt, rv;
 };
 
 __device__ __forceinline__ bool inner(int x, int y, int z, int nx, int ny, int nz) {
@@ -483,7 +485,8 @@ __device__ __forceinline__ bool inner(int x, int y, int z, int nx, int ny, int n
 
 // Per coarse cell: Favre-filtered velocity, filtered rho, phi and strain rate
 // (fine S = -3 Pi / (2 rho tau), tau = 1/2 + 3 nu(phi)), and the subgrid
-// stress T = <rho u u> - <rho> u~ u~.  Ghost cells: rho 1, the rest 0.
+// stress T = <rho u u> - <rho> u~ u~, all in coarse lattice units (velocity
+// x rv, strain rate x rt, stress x rv^2).  Ghost cells: rho 1, the rest 0.
 __global__ void __launch_bounds__(256) k_filter(Fine f, int cnx, int cny, int cnz, float *__restrict__ out) {
     const size_t nc = size_t(cnx) * cny * cnz;
     const size_t c = size_t(blockIdx.x) * 256u + threadIdx.x;
@@ -520,15 +523,16 @@ __global__ void __launch_bounds__(256) k_filter(Fine f, int cnx, int cny, int cn
                 }
         const float rb = r * 0.125f, ut[3] = {mu[0] / r, mu[1] / r, mu[2] / r};
         v[kRho] = rb;
-        v[kUx] = ut[0];
-        v[kUy] = ut[1];
-        v[kUz] = ut[2];
+        v[kUx] = f.rv * ut[0];
+        v[kUy] = f.rv * ut[1];
+        v[kUz] = f.rv * ut[2];
         v[kPhi] = ph * 0.125f;
         const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+        const float rv2 = f.rv * f.rv;
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
-            v[kS + a] = 2.0f * st[a] * 0.125f;
-            v[kT + a] = muu[a] * 0.125f - rb * ut[ia[a]] * ut[ib[a]];
+            v[kS + a] = f.rt * st[a] * 0.125f;
+            v[kT + a] = rv2 * (muu[a] * 0.125f - rb * ut[ia[a]] * ut[ib[a]]);
         }
     }
 #pragma unroll
@@ -695,6 +699,13 @@ int ssmd_sgs_samples(const ssmd_params *fine, const ssmd_fields *fd, const ssmd_
     f.phi = fd->phi;
     f.visc_water = float(fine->visc_water);
     f.visc_diff = float(fine->visc_oil) - float(fine->visc_water);
+    // The two runs describe one physical flow with dx_c = 2 dx_f, so equal
+    // physical viscosity fixes the time ratio: nu_c = nu_f rt / 4.
+    if (!(fine->visc_water > 0.0) || !(coarse->visc_water > 0.0)) return SSMD_EINVAL;
+    const double rt = 4.0 * coarse->visc_water / fine->visc_water;
+    if (!(rt >= 1.0 && rt <= 16.0)) return SSMD_EINVAL;
+    f.rt = float(rt);
+    f.rv = float(0.5 * rt);
     const size_t nc = size_t(coarse->nx) * coarse->ny * coarse->nz;
     const dim3 grid(unsigned((nc + 255) / 256)), block(256);
     hipStream_t s = static_cast<hipStream_t>(stream);

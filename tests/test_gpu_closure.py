@@ -247,19 +247,24 @@ def test_learned_closure_graph_replay_matches_eager(gpu):
 
 
 # -- training data from a filtered fine field set -----------------------------
-def test_sgs_samples_analytic(gpu):
+@pytest.mark.parametrize("scaling", ["acoustic", "diffusive"])
+def test_sgs_samples_analytic(gpu, scaling):
     """Linear fields on the fine grid give closed forms after the 2x box filter:
-    u_x = a y, phi = b x, Pi = diag(p0, -p0, 0), rho = 1:
-      u~_x = a (2 cy - 1/2), phi~ = b (2 cx - 1/2), S = 2 (-3 Pi / (2 tau_f)),
-      T_xx = a^2 / 4 (variance of y over a block), other T = 0,
-      nu_t = -(dev T : S) / (2 S:S) = a^2 / (16 s) with s = 3 p0 / tau_f."""
+    u_x = a y, phi = b x, Pi = diag(p0, -p0, 0), rho = 1; in coarse lattice
+    units, with rt = dt_c / dt_f = 4 nu_c / nu_f (2 acoustic, 4 diffusive) and
+    rv = rt / 2:
+      u~_x = rv a (2 cy - 1/2), phi~ = b (2 cx - 1/2), S = rt (-3 Pi / (2 tau_f)),
+      T_xx = rv^2 a^2 / 4 (variance of y over a block), other T = 0,
+      nu_t = -(dev T : S) / (2 S:S) = rv^2 a^2 / (16 s) with s = 1.5 rt p0 / tau_f."""
     from ssmdlbm_amd.closure_study import Filter
     coarse = cases.JET_D3Q19_SMALL.with_(nx=12, ny=10, nz=14, diameter=4).params()
     coarse.oil.enabled = 0  # only the geometry matters here; no inlet near the periodic faces
     coarse.closure = abi.CLOSURE_LEARNED
     coarse.strain_scale, coarse.grad_scale, coarse.tau_scale = 1000.0, 25.0, 0.01
     fine = cases.JET_D3Q19_SMALL.with_(nx=22, ny=18, nz=26, diameter=4).params()
-    nu = 0.1 / 3.0
+    nu = 2.0 * coarse.visc_water if scaling == "acoustic" else coarse.visc_water
+    rt = 2.0 if scaling == "acoustic" else 4.0
+    rv = rt / 2
     fine.visc_water = fine.visc_oil = nu
     tau_f = 0.5 + 3 * nu
     a, b, p0 = 0.01, 0.02, 1e-4
@@ -279,21 +284,21 @@ def test_sgs_samples_analytic(gpu):
     flt(sim)
     torch.cuda.synchronize()
     cz, cy, cx = coarse.shape
-    s = 3 * p0 / tau_f
+    s = 1.5 * rt * p0 / tau_f
     inner = (slice(1, cz - 1), slice(1, cy - 1), slice(1, cx - 1))
     Z, Y, X = np.meshgrid(np.arange(cz), np.arange(cy), np.arange(cx), indexing="ij")
     pl = lambda k: flt.plane(k).cpu().double().numpy()
     np.testing.assert_allclose(pl(0)[inner], 1.0, rtol=1e-6)
-    np.testing.assert_allclose(pl(1)[inner], (a * (2 * Y - 0.5))[inner], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(pl(1)[inner], (rv * a * (2 * Y - 0.5))[inner], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(pl(4)[inner], (b * (2 * X - 0.5))[inner], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(pl(5)[inner], -s, rtol=1e-5)
     np.testing.assert_allclose(pl(6)[inner], s, rtol=1e-5)
-    np.testing.assert_allclose(pl(11)[inner], a * a / 4, rtol=2e-3)  # a difference of nearby squares in FP32
+    np.testing.assert_allclose(pl(11)[inner], rv * rv * a * a / 4, rtol=2e-3)  # nearby squares differ in FP32
     for k in (7, 8, 9, 10, 12, 13, 14, 15, 16):
-        assert np.abs(pl(k)[inner]).max() < 1e-5 * a * a, k
+        assert np.abs(pl(k)[inner]).max() < 1e-5 * rv * rv * a * a, k
     target = flt.target.cpu().double().numpy().reshape(coarse.shape)
     weight = flt.weight.cpu().double().numpy().reshape(coarse.shape)
-    np.testing.assert_allclose(target[inner], 3 * a * a / (16 * s) / 0.01, rtol=3e-3)
+    np.testing.assert_allclose(target[inner], 3 * rv * rv * a * a / (16 * s) / 0.01, rtol=3e-3)
     np.testing.assert_allclose(weight[inner], 2 * s * s, rtol=1e-5)
     ghost = np.ones(coarse.shape, dtype=bool)
     ghost[inner] = False
@@ -308,14 +313,15 @@ def test_sgs_samples_analytic(gpu):
     np.testing.assert_allclose(feat[deep][..., 7], 2 * b * 25.0, rtol=2 ** -8)
 
 
-def test_learned_closure_beats_no_closure_and_smagorinsky_a_priori(gpu):
+@pytest.mark.parametrize("scaling", ["diffusive", "acoustic"])
+def test_learned_closure_beats_no_closure_and_smagorinsky_a_priori(gpu, scaling):
     """Filtered 64x64x128 D3Q19 jet -> 32x32x64 samples: a network trained on
     earlier snapshots predicts the subgrid stress of later (held-out)
     snapshots better than no closure (tau_t = 0) and than Smagorinsky
     (tau_t = 3 Cs^2 |S|, Cs = 0.1), in the weighted squared error whose
     minimum is the least-squares fit of the subgrid stress."""
     from ssmdlbm_amd.closure_study import Filter, Geometry, Jet, SampleSet, coarse_params, fine_params
-    g = Geometry(nx=32, ny=32, nz=64, q=19, diameter=8, tau=0.55)
+    g = Geometry(nx=32, ny=32, nz=64, q=19, diameter=8, tau=0.55, scaling=scaling)
     jet = Jet(u=0.05, sigma=0.05, seed=3)
     sc = Scales(strain_scale=100.0, grad_scale=4.0, tau_scale=0.01)
     fp, cp = fine_params(g, jet), coarse_params(g, jet, abi.CLOSURE_LEARNED, sc)
@@ -324,7 +330,7 @@ def test_learned_closure_beats_no_closure_and_smagorinsky_a_priori(gpu):
     flt = Filter(fp, cp)
     train, held = SampleSet(1 << 20, seed=1), SampleSet(1 << 20, seed=2)
     for k, m in enumerate(range(600, 1601, 100)):
-        sim.graph_run(2 * m - 1 - sim.step_index)
+        sim.graph_run(g.fine_steps * m - 1 - sim.step_index)
         sim.step(1, macros=True)
         flt(sim)
         (train if k < 8 else held).add(flt)
