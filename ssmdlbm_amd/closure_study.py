@@ -29,6 +29,7 @@ Command line (one GPU):
   python -m ssmdlbm_amd.closure_study train --out DIR   fine runs -> network
   python -m ssmdlbm_amd.closure_study eval  --net DIR/closure --out DIR
   python -m ssmdlbm_amd.closure_study noise --out DIR   seed-to-seed spread
+  python -m ssmdlbm_amd.closure_study sensitivity --report DIR/eval_report.json --out DIR
 """
 from __future__ import annotations
 
@@ -260,8 +261,12 @@ def run_fine(g: Geometry, jet: Jet, coarse_steps: int, sample_every: int, stat_f
 
 
 def run_coarse(g: Geometry, jet: Jet, closure: int, net: Network | None, coarse_steps: int, sample_every: int,
-               stat_from: int, z_range, kmax, tune: bool = True) -> dict:
+               stat_from: int, z_range, kmax, tune: bool = True, edit=None) -> dict:
+    """Coarse run and its statistics; `edit(params)` may change the
+    parameters before the run (sensitivity study)."""
     p = coarse_params(g, jet, closure, net.scales if net else None)
+    if edit is not None:
+        edit(p)
     sim = Simulation(p)
     if closure == abi.CLOSURE_LEARNED:
         net.load_into(sim)
@@ -469,10 +474,47 @@ def noise_main(args) -> int:
     return 0
 
 
+def sensitivity_main(args) -> int:
+    """Which coarse-grid parameter the a-posteriori errors respond to: the
+    no-closure coarse jet of an evaluation report with its viscosity
+    (tau - 1/2), surface tension or sharpening coefficient scaled by each
+    factor, scored against that report's filtered fine statistics."""
+    out = Path(args.out)
+    out.mkdir(parents=True, exist_ok=True)
+    ref = json.loads(Path(args.report).read_text())
+    geo = dict(ref["geometry"])
+    g = Geometry(**geo)
+    jet = Jet(**ref["jet"])
+    z_range, kmax, stat_from = tuple(ref["z_range"]), ref["kmax"], ref["stat_from"]
+    fine = ref["filtered_fine"]
+    tau0 = g.tau
+
+    def scaled(name, f):
+        def edit(p):
+            if name == "viscosity":
+                p.visc_water = p.visc_oil = viscosity_from_tau(0.5 + f * (tau0 - 0.5))
+            elif name == "sigma":
+                p.sigma = f * jet.sigma
+            elif name == "gamma":
+                p.gamma = f * p.gamma
+        return edit
+
+    res = {"report": str(args.report), "factors": args.factors, "runs": {}}
+    for name in ("viscosity", "sigma", "gamma"):
+        for f in args.factors:
+            r = run_coarse(g, jet, abi.CLOSURE_NONE, None, ref["steps"], ref["every"], stat_from, z_range, kmax,
+                           edit=scaled(name, f))
+            e = compare(r, fine, z_range, kmax) if r["finite"] else None
+            res["runs"][f"{name} x{f}"] = e
+            log(f"{name} x{f}: {e}")
+    (out / "sensitivity_report.json").write_text(json.dumps(res, indent=1))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
-    for name in ("train", "eval", "noise"):
+    for name in ("train", "eval", "noise", "sensitivity"):
         s = sub.add_parser(name)
         s.add_argument("--out", required=True)
         s.add_argument("--nx", type=int, default=128)
@@ -505,11 +547,15 @@ def main(argv=None) -> int:
     n.add_argument("--every", type=int, default=50)
     n.add_argument("--kmax", type=int, default=32)
     n.add_argument("--seeds", type=int, default=4)
+    sn = sub.choices["sensitivity"]
+    sn.add_argument("--report", required=True, help="an eval_report.json (its geometry, jet and fine statistics)")
+    sn.add_argument("--factors", type=float, nargs="+", default=[0.5, 0.8, 1.25, 2.0])
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("the closure study runs on the GPU", file=sys.stderr)
         return 1
-    return {"train": train_main, "eval": eval_main, "noise": noise_main}[args.cmd](args)
+    return {"train": train_main, "eval": eval_main, "noise": noise_main,
+            "sensitivity": sensitivity_main}[args.cmd](args)
 
 
 if __name__ == "__main__":
