@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SSMD_ABI_VERSION 3
+#define SSMD_ABI_VERSION 4
 
 #define SSMD_OK 0
 #define SSMD_EINVAL (-1)       /* bad argument or inconsistent parameters */
@@ -119,10 +119,14 @@ typedef struct ssmd_params {
     /* LEARNED: network inputs per cell, in order: the strain rate
      * S_ab = -3 Pi_ab / (2 rho tau) times strain_scale (xx, yy, zz, xy, xz, yz),
      * phi, and |grad phi-hat| (the masked gradient of the force) times
-     * grad_scale; output tau_t = tau_scale * max(0, network(inputs)). */
+     * grad_scale; output tau_t = max(tau_floor, tau_scale * network(inputs)).
+     * tau_floor = 0 (default) keeps tau_t >= 0 (no backscatter); a negative
+     * floor admits backscatter and must keep tau above 1/2 with margin:
+     * tau_floor >= -0.9 * 3 * min(visc_water, visc_oil), or SSMD_EINVAL. */
     double strain_scale;
     double grad_scale;
     double tau_scale;
+    double tau_floor;
 } ssmd_params;
 
 /* Device buffers.  Member order follows struct LBMFields (LBMFields.cuh:26-85);
@@ -347,7 +351,7 @@ int ssmd_mlp_adam(float *params, const float *grad, float *m, float *v, uint32_t
 size_t ssmd_sgs_work_bytes(const ssmd_params *coarse);
 int ssmd_sgs_samples(const ssmd_params *fine, const ssmd_fields *fd, const ssmd_params *coarse, float *work,
                      uint16_t *feat, float *target, float *weight, void *stream);
-/* LEARNED closure: tau_t[c] = tau_scale * max(0, network(feat[c])) for every
+/* LEARNED closure: tau_t[c] = max(tau_floor, tau_scale * network(feat[c])) for every
  * cell (ssmd_step and the step graphs call it after the boundaries). */
 int ssmd_closure_update(const ssmd_params *p, const ssmd_fields *d, void *stream);
 

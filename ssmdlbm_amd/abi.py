@@ -14,7 +14,7 @@ POP_F32, POP_F16, POP_BF16 = 0, 1, 2
 WRAP_NONE, WRAP_REFERENCE, WRAP_TRUE = 0, 1, 2
 CLOSURE_NONE, CLOSURE_SMAGORINSKY, CLOSURE_LEARNED = 0, 1, 2
 MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 8, 64, 8961
-ABI_VERSION = 3
+ABI_VERSION = 4
 STEP_MACROS = 1
 REDUCE_DOUBLES = 8192
 POP_BYTES = {POP_F32: 4, POP_F16: 2, POP_BF16: 2}
@@ -60,6 +60,7 @@ class Params(C.Structure):
         ("strain_scale", C.c_double),
         ("grad_scale", C.c_double),
         ("tau_scale", C.c_double),
+        ("tau_floor", C.c_double),
     ]
 
     @property

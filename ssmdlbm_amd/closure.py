@@ -49,13 +49,16 @@ def bits_to_float(b: torch.Tensor) -> torch.Tensor:
 
 @dataclass
 class Scales:
-    """Input/output scaling the lattice kernel applies (ssmd_params fields)."""
+    """Input/output scaling the lattice kernel applies (ssmd_params fields):
+    tau_t = max(tau_floor, tau_scale * network); tau_floor < 0 admits backscatter."""
     strain_scale: float = 1.0
     grad_scale: float = 1.0
     tau_scale: float = 1.0
+    tau_floor: float = 0.0
 
     def apply(self, p: abi.Params) -> None:
         p.strain_scale, p.grad_scale, p.tau_scale = self.strain_scale, self.grad_scale, self.tau_scale
+        p.tau_floor = self.tau_floor
 
 
 def init_params(seed: int = 0) -> np.ndarray:

@@ -30,6 +30,7 @@ Command line (one GPU):
   python -m ssmdlbm_amd.closure_study eval  --net DIR/closure --out DIR
   python -m ssmdlbm_amd.closure_study noise --out DIR   seed-to-seed spread
   python -m ssmdlbm_amd.closure_study sensitivity --report DIR/eval_report.json --out DIR
+  python -m ssmdlbm_amd.closure_study calibrate --report DIR/eval_report.json --out DIR
 """
 from __future__ import annotations
 
@@ -46,7 +47,7 @@ import torch
 
 from . import abi
 from .cases import gamma_from_tau_g, viscosity_from_tau
-from .closure import Network, Scales
+from .closure import Network, Scales, offsets
 from .solver import Simulation
 
 
@@ -511,10 +512,74 @@ def sensitivity_main(args) -> int:
     return 0
 
 
+def interface_network(kappa: float, beta: float) -> np.ndarray:
+    """Parameters of the network y = kappa * x7 + beta, x7 = |grad phi-hat| *
+    grad_scale >= 0: hidden unit 0 of each layer carries x7 through the ReLUs
+    (weights 1, exact in bf16), the output layer scales and offsets it."""
+    p = np.zeros(abi.MLP_PARAMS, dtype=np.float32)
+    o = offsets()
+    p[o["W1"][0] + 0 * abi.MLP_INPUTS + 7] = 1.0
+    p[o["W2"][0] + 0 * abi.MLP_HIDDEN + 0] = 1.0
+    p[o["W3"][0] + 0 * abi.MLP_HIDDEN + 0] = 1.0
+    p[o["W4"][0] + 0] = kappa
+    p[o["b4"][0]] = beta
+    return p
+
+
+def calibrate_main(args) -> int:
+    """A-posteriori calibration of a signed, interface-localised relaxation
+    time tau_t = max(floor, 1e-3 (kappa x7 + beta)): Nelder-Mead on (kappa,
+    beta) minimising the largest of the three statistic errors of a coarse
+    LEARNED run of a training jet against that jet's filtered fine run; the
+    result is scored on the held-out jet of an evaluation report."""
+    from scipy.optimize import minimize
+
+    out = Path(args.out)
+    out.mkdir(parents=True, exist_ok=True)
+    ref = json.loads(Path(args.report).read_text())
+    g = Geometry(**dict(ref["geometry"]))
+    held = Jet(**ref["jet"])
+    z_range, kmax, stat_from, steps, every = tuple(ref["z_range"]), ref["kmax"], ref["stat_from"], ref["steps"], ref["every"]
+    train = Jet(args.train_u, args.train_sigma, args.seed)
+    floor = -0.9 * 3.0 * viscosity_from_tau(g.tau) * 0.99
+    log(f"training jet {asdict(train)}: filtered fine statistics")
+    fine_train = run_fine(g, train, steps, every, stat_from, [], z_range, kmax)
+    history = []
+
+    def errors(jet, fine, kappa, beta):
+        net = Network(interface_network(kappa, beta),
+                      scales=Scales(strain_scale=1.0, grad_scale=args.grad_scale, tau_scale=1e-3, tau_floor=floor))
+        r = run_coarse(g, jet, abi.CLOSURE_LEARNED, net, steps, every, stat_from, z_range, kmax, tune=False)
+        e = compare(r, fine, z_range, kmax) if r["finite"] else None
+        return e, r
+
+    def objective(v):
+        e, r = errors(train, fine_train, float(v[0]), float(v[1]))
+        j = max(e.values()) if e else 10.0
+        history.append({"kappa": float(v[0]), "beta": float(v[1]), "errors": e, "objective": j,
+                        "tau_t_mean": r.get("tau_t_mean"), "tau_t_max": r.get("tau_t_max")})
+        log(f"kappa {v[0]:8.3f} beta {v[1]:8.3f}: max error {j:.4f} {e}")
+        return j
+
+    res = minimize(objective, x0=np.array([0.0, 0.0]), method="Nelder-Mead",
+                   options={"maxfev": args.evals, "initial_simplex": np.array([[0.0, 0.0], [args.step, 0.0],
+                                                                             [0.0, args.step]])})
+    kappa, beta = float(res.x[0]), float(res.x[1])
+    held_fine = ref["filtered_fine"]
+    e_best, _ = errors(held, held_fine, kappa, beta)
+    e_none = ref["errors"]["none"]
+    rep = {"train_jet": asdict(train), "held_out_jet": asdict(held), "floor": floor, "grad_scale": args.grad_scale,
+           "history": history, "best": {"kappa": kappa, "beta": beta, "train_objective": float(res.fun)},
+           "held_out": {"calibrated": e_best, "none": e_none}}
+    (out / "calibrate_report.json").write_text(json.dumps(rep, indent=1))
+    log(f"best kappa {kappa:.3f} beta {beta:.3f}; held-out calibrated {e_best}; none {e_none}")
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
-    for name in ("train", "eval", "noise", "sensitivity"):
+    for name in ("train", "eval", "noise", "sensitivity", "calibrate"):
         s = sub.add_parser(name)
         s.add_argument("--out", required=True)
         s.add_argument("--nx", type=int, default=128)
@@ -550,12 +615,19 @@ def main(argv=None) -> int:
     sn = sub.choices["sensitivity"]
     sn.add_argument("--report", required=True, help="an eval_report.json (its geometry, jet and fine statistics)")
     sn.add_argument("--factors", type=float, nargs="+", default=[0.5, 0.8, 1.25, 2.0])
+    ca = sub.choices["calibrate"]
+    ca.add_argument("--report", required=True, help="the held-out jet's eval_report.json")
+    ca.add_argument("--train-u", type=float, default=0.06)
+    ca.add_argument("--train-sigma", type=float, default=0.015)
+    ca.add_argument("--grad-scale", type=float, default=4.0)
+    ca.add_argument("--step", type=float, default=10.0, help="initial simplex step (units of 1e-3 in tau)")
+    ca.add_argument("--evals", type=int, default=30)
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("the closure study runs on the GPU", file=sys.stderr)
         return 1
-    return {"train": train_main, "eval": eval_main, "noise": noise_main,
-            "sensitivity": sensitivity_main}[args.cmd](args)
+    return {"train": train_main, "eval": eval_main, "noise": noise_main, "sensitivity": sensitivity_main,
+            "calibrate": calibrate_main}[args.cmd](args)
 
 
 if __name__ == "__main__":

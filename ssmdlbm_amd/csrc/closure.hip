@@ -142,10 +142,11 @@ __device__ __forceinline__ bf16x8 load_x(const uint16_t *__restrict__ x, uint32_
 
 // --------------------------------------------------------------------------
 // Inference: each wavefront keeps the whole network in registers and walks
-// 32-cell tiles.  closure != 0 writes scale * max(0, y) (the eddy
+// 32-cell tiles.  closure != 0 writes max(floor, scale * y) (the eddy
 // relaxation time), else y.
 __global__ void __launch_bounds__(256) k_forward(const uint8_t *__restrict__ packed, const uint16_t *__restrict__ x,
-                                                 float *__restrict__ y, uint32_t n, float scale, int closure) {
+                                                 float *__restrict__ y, uint32_t n, float scale, float floor,
+                                                 int closure) {
     const int lane = int(threadIdx.x) & 63;
     const bf16x8 *A1 = reinterpret_cast<const bf16x8 *>(packed + off::A1);
     const bf16x8 *A2 = reinterpret_cast<const bf16x8 *>(packed + off::A2);
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(256) k_forward(const uint8_t *__restrict__ pac
         chunks<true>(c, b);
         layer64([&](int mb, int kc) { return a3[mb][kc]; }, b, b3, c);
         const float o = out_dot(c, w4, b4);
-        if (lane < 32 && ok) y[cell] = closure ? scale * __builtin_fmaxf(o, 0.f) : o;
+        if (lane < 32 && ok) y[cell] = closure ? __builtin_fmaxf(scale * o, floor) : o;
     }
 }
 
@@ -629,7 +630,7 @@ int ssmd_mlp_forward(const void *packed, const uint16_t *x, float *y, uint32_t n
     uint32_t blocks = (tiles + 3) / 4;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_forward, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const uint8_t *>(packed), x, y, n, 1.0f, 0);
+                       static_cast<const uint8_t *>(packed), x, y, n, 1.0f, 0.0f, 0);
     return herr(hipGetLastError());
 }
 
@@ -725,14 +726,14 @@ int ssmd_sgs_samples(const ssmd_params *fine, const ssmd_fields *fd, const ssmd_
 }  // extern "C"
 
 namespace ssmd {
-// tau_t = scale * max(0, network(feat)) over n cells (the LEARNED closure).
+// tau_t = max(floor, scale * network(feat)) over n cells (the LEARNED closure).
 int launch_closure_update(const void *packed, const uint16_t *feat, float *tau_t, uint32_t n, float scale,
-                          hipStream_t s) {
+                          float floor, hipStream_t s) {
     const uint32_t tiles = (n + 31) / 32;
     uint32_t blocks = (tiles + 3) / 4;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(mlp::k_forward, dim3(blocks), dim3(256), 0, s, static_cast<const uint8_t *>(packed), feat,
-                       tau_t, n, scale, 1);
+                       tau_t, n, scale, floor, 1);
     return herr(hipGetLastError());
 }
 }  // namespace ssmd

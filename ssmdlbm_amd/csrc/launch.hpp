@@ -44,9 +44,9 @@ template <int Q, class P>
 hipError_t launch_lattice_batch(int tile, bool macros, const JetSlot *jets, int njets, uint32_t total_tiles,
                                 hipStream_t s);
 
-// LEARNED closure pass: tau_t = scale * max(0, network(feat)) over n cells
-// (closure.hip).
+// LEARNED closure pass: tau_t = max(floor, scale * network(feat)) over n
+// cells (closure.hip).
 int launch_closure_update(const void *packed, const uint16_t *feat, float *tau_t, uint32_t n, float scale,
-                          hipStream_t s);
+                          float floor, hipStream_t s);
 
 }  // namespace ssmd

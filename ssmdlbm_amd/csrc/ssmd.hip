@@ -8,6 +8,7 @@
 // Errors are returned, never aborted on.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -229,7 +230,8 @@ int step_dispatch(const ssmd_params &p, const ssmd_fields &d, uint64_t *counter,
     SSMD_TRY(launch_lattice(p, d, flags, s));
     SSMD_TRY(launch_boundaries(p, d, counter, step, flags, s));
     if (p.closure == SSMD_CLOSURE_LEARNED)
-        return launch_closure_update(d.mlp, d.feat, d.tau_t, p.nx * p.ny * p.nz, float(p.tau_scale), s);
+        return launch_closure_update(d.mlp, d.feat, d.tau_t, p.nx * p.ny * p.nz, float(p.tau_scale),
+                                     float(p.tau_floor), s);
     return SSMD_OK;
 }
 
@@ -344,6 +346,9 @@ int ssmd_validate(const ssmd_params *p) {
     if (p->closure == SSMD_CLOSURE_SMAGORINSKY && !(p->cs >= 0.0)) return SSMD_EINVAL;
     if (p->closure == SSMD_CLOSURE_LEARNED &&
         !(p->tau_scale >= 0.0 && std::isfinite(p->strain_scale) && std::isfinite(p->grad_scale)))
+        return SSMD_EINVAL;
+    // tau = 1/2 + 3 nu(phi) + tau_t must stay above 1/2 with margin
+    if (!(p->tau_floor <= 0.0 && p->tau_floor >= -0.9 * 3.0 * std::min(p->visc_water, p->visc_oil)))
         return SSMD_EINVAL;
     return SSMD_OK;
 }
@@ -491,7 +496,7 @@ int ssmd_closure_update(const ssmd_params *p, const ssmd_fields *d, void *stream
     if (p->closure != SSMD_CLOSURE_LEARNED) return SSMD_EINVAL;
     if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
     return launch_closure_update(d->mlp, d->feat, d->tau_t, p->nx * p->ny * p->nz, float(p->tau_scale),
-                                 static_cast<hipStream_t>(stream));
+                                 float(p->tau_floor), static_cast<hipStream_t>(stream));
 }
 
 int ssmd_launch_phase(const ssmd_params *p, const ssmd_fields *d, void *stream) {

@@ -40,7 +40,7 @@ int main(void) {
          offsetof(ssmd_params, water), offsetof(ssmd_params, oil));
   printf("%zu %zu %zu\\n", offsetof(ssmd_fields, f), offsetof(ssmd_fields, step_counter), offsetof(ssmd_fields, owned));
   printf("%zu %zu %zu %zu\\n", offsetof(ssmd_params, closure), offsetof(ssmd_params, cs),
-         offsetof(ssmd_params, tau_scale), offsetof(ssmd_fields, mlp));
+         offsetof(ssmd_params, tau_floor), offsetof(ssmd_fields, mlp));
   printf("%d %d %d\\n", SSMD_ABI_VERSION, SSMD_MLP_PARAMS, SSMD_CLOSURE_LEARNED);
   return 0;
 }
@@ -53,7 +53,7 @@ int main(void) {
     assert out[1] == [abi.Params.visc_water.offset, abi.Params.seed.offset, abi.Params.water.offset,
                       abi.Params.oil.offset]
     assert out[2] == [abi.Fields.f.offset, abi.Fields.step_counter.offset, abi.Fields.owned.offset]
-    assert out[3] == [abi.Params.closure.offset, abi.Params.cs.offset, abi.Params.tau_scale.offset,
+    assert out[3] == [abi.Params.closure.offset, abi.Params.cs.offset, abi.Params.tau_floor.offset,
                       abi.Fields.mlp.offset]
     assert out[4] == [abi.ABI_VERSION, abi.MLP_PARAMS, abi.CLOSURE_LEARNED]
 
@@ -112,6 +112,12 @@ def test_validation_rejects_bad_parameters():
     b = p.copy()
     b.pop_format = abi.POP_F16
     assert lib.ssmd_bytes(b, 1) == 19 * 2 * p.cells and lib.ssmd_bytes(b, 2) == 4 * 4 * p.cells
+    # the LEARNED floor keeps tau = 1/2 + 3 nu + tau_t above 1/2 with a 10% margin
+    margin = 0.9 * 3 * min(p.visc_water, p.visc_oil)
+    for floor, rc in ((0.0, 0), (-0.99 * margin, 0), (-1.01 * margin, -1), (1e-3, -1), (float("nan"), -1)):
+        b = p.copy()
+        b.tau_floor = floor
+        assert lib.ssmd_validate(b) == rc, floor
 
 
 def test_error_paths_without_device():

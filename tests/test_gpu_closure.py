@@ -217,13 +217,21 @@ def test_learned_closure_inputs_and_update(gpu):
         scale = np.abs(ref).max()
         bad = np.abs(got - ref) > 2.0 ** -8 * np.abs(ref) + 1e-5 * scale
         assert scale > 0 and not bad.any(), (j, int(bad.sum()), scale)
-    # the network pass: tau_t = tau_scale * max(0, network(inputs))
+    # the network pass: tau_t = max(tau_floor, tau_scale * network(inputs)), floor 0
     sim.closure_update()
     got = sim.field("tau_t").reshape(-1)
     yo = O.mlp_forward(params.astype(np.float64), feat, quant=True)
     want = 0.02 * np.maximum(yo, 0.0)
     rms = np.sqrt(np.mean(want ** 2))
     assert rms > 0
+    assert np.abs(got - want).max() <= 1e-2 * rms + 1e-9
+    # a negative floor admits backscatter down to the floor
+    assert (yo < 0).any()
+    sim.p.tau_floor = -0.003
+    sim.closure_update()
+    got = sim.field("tau_t").reshape(-1)
+    want = np.maximum(0.02 * yo, -0.003)
+    assert got.min() < 0 and got.min() >= np.float32(-0.003)  # the floor as the kernel holds it
     assert np.abs(got - want).max() <= 1e-2 * rms + 1e-9
 
 

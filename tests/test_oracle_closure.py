@@ -147,3 +147,19 @@ def test_weighted_loss_is_linear_in_the_weights(problem):
     n, m = len(t), int(keep.sum())
     assert lz * n == pytest.approx(lk * m, rel=1e-12)
     assert np.allclose(gz * n, gk * m, rtol=1e-10, atol=1e-14)
+
+
+def test_interface_network_is_affine_in_the_gradient_feature():
+    """closure_study.interface_network(kappa, beta) is the network
+    y = kappa * x7 + beta (x7 = |grad phi-hat| * grad_scale >= 0), exactly,
+    through the bf16 quantization of the matrix-core kernels."""
+    from oracle.oracle import mlp_forward
+    from ssmdlbm_amd.closure_study import interface_network
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((64, 8))
+    # bf16-representable gradient features, as the lattice kernel writes them
+    x[:, 7] = (np.abs(x[:, 7]).astype(np.float32).view(np.uint32) & 0xFFFF0000).view(np.float32)
+    for kappa, beta in ((12.5, -3.0), (-10.62, -1.52), (0.0, 0.25)):
+        y = mlp_forward(interface_network(kappa, beta).astype(np.float64), x, quant=True)
+        k32, b32 = float(np.float32(kappa)), float(np.float32(beta))  # the parameters are float32
+        np.testing.assert_allclose(y, k32 * x[:, 7] + b32, rtol=1e-12, atol=1e-12)
