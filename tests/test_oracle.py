@@ -18,7 +18,7 @@ import pytest
 from conftest import cpu_threads
 from oracle.oracle import Oracle, load
 from ssmdlbm_amd import abi, cases
-from ssmdlbm_amd.solver import arrival_to_source, source_to_arrival
+from ssmdlbm_amd.solver import arrival_to_source, phase_moments, phase_populations, source_to_arrival
 
 GOLD = json.loads((Path(__file__).parent / "golden" / "velocity_sets.json").read_text())
 RNG = np.random.default_rng(1234)
@@ -398,3 +398,28 @@ def test_layout_conversion_roundtrip():
     np.testing.assert_array_equal(a[ok], b[ok])
     # arrivals whose source is outside are exactly the NaNs
     assert np.isnan(b[1][:, :, 0]).all() and not np.isnan(b[1][:, :, 1:]).any()
+
+
+def test_phase_moment_form_round_trip_and_rejection(oracle_lib):
+    """The 4-plane moment form of the D3Q7 populations: seven populations
+    w_k (g_phi + c_k . o) round-trip exactly; the oracle's own post-collision
+    populations (equilibrium plus sharpening, reference D3Q7.cuh:124-148) have
+    that form; a population set outside it is rejected."""
+    rng = np.random.default_rng(8)
+    mom = rng.standard_normal((4, 3, 4, 5)).astype(np.float32).astype(np.float64)
+    # the populations in exact (double) arithmetic invert exactly
+    pops = np.stack([0.25 * mom[0]] + [0.125 * (mom[0] + sg * mom[1 + a]) for a in range(3) for sg in (1, -1)])
+    np.testing.assert_array_equal(phase_moments(pops), mom)
+    # and phase_populations forms them with one FP32 rounding each
+    np.testing.assert_array_equal(phase_populations(mom), pops.astype(np.float32))
+    # the oracle's g_eq + anti-diffusion of one cell: g_phi = phi, o = 4 phi u + sharp n
+    phi, u, n = 0.3, np.array([0.01, -0.02, 0.03]), np.array([0.6, 0.0, 0.8])
+    sharp = 0.4 * phi * (1 - phi)
+    g = np.array([oracle_lib.oracle_geq(k, phi, *u) + oracle_lib.oracle_antidiff(k, sharp, *n) for k in range(7)])
+    m = phase_moments(g.reshape(7, 1, 1, 1))
+    assert m[0].item() == pytest.approx(phi, rel=1e-12)
+    np.testing.assert_allclose(m[1:, 0, 0, 0], 4 * phi * u + sharp * n, rtol=1e-12)
+    bad = pops.copy()
+    bad[1, 1, 1, 1] += 0.01  # one population off the family
+    with pytest.raises(ValueError):
+        phase_moments(bad)
