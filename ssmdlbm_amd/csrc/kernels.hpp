@@ -644,13 +644,24 @@ __global__ void __launch_bounds__(256) k_phase_batch(const JetSlot *__restrict__
 #endif  // SSMD_STEP_KERNELS
 
 
-// Persistent lattice kernel: gridDim.x blocks stride over all tiles of all jets.
+// Persistent lattice kernel: gridDim.x resident blocks take the tiles of all
+// jets from a queue, in tile order (queue[0] = next ticket), so blocks that
+// drew cheap tiles take more of them.  A block leaves when its ticket is past
+// the last tile; the last block to leave (queue[1] counts them) resets both
+// words, so every launch -- a graph replay included -- starts from 0.
 template <int Q, class P, int TX, int TY, bool MACROS>
 __global__ void __launch_bounds__(TX *TY) k_lattice_batch(const JetSlot *__restrict__ jets, int njets,
-                                                          uint32_t total_tiles) {
+                                                          uint32_t total_tiles, uint32_t *__restrict__ queue) {
     __shared__ LatticeSmem<TX, TY> sm;
+    __shared__ uint32_t ticket;
     using PT = typename P::T;
-    for (uint32_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+    for (;;) {
+        __syncthreads();  // the previous tile's LDS readers and ticket readers are done
+        if (lead) ticket = __hip_atomic_fetch_add(&queue[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t t = ticket;
+        if (t >= total_tiles) break;
         const int j = find_slot(jets, njets, t, 0);
         const JetSlot &js = jets[j];
         uint32_t lt = t - js.tile_begin;
@@ -659,10 +670,15 @@ __global__ void __launch_bounds__(TX *TY) k_lattice_batch(const JetSlot *__restr
         const int by = int(lt % uint32_t(js.tiles_y));
         const int bz = int(lt / uint32_t(js.tiles_y));
         const int zb = 1 + bz * js.lz;
-        __syncthreads();  // the previous tile's LDS readers are done
         lattice_tile<Q, P, TX, TY, MACROS>(js.k, static_cast<const PT *>(js.f), static_cast<PT *>(js.f_next),
                                            js.g_next, js.phi, js.mac, bx * TX, by * TY, zb,
                                            min(zb + js.lz, js.k.nz - 1), sm);
+    }
+    if (lead &&
+        __hip_atomic_fetch_add(&queue[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+        // every block has drawn its last ticket: reset for the next launch
+        __hip_atomic_store(&queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&queue[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

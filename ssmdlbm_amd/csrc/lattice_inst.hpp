@@ -19,7 +19,7 @@ hipError_t lattice_one(const LatticeArgs &a, hipStream_t s) {
 }
 
 template <int Q, class P, int TX, int TY, bool MACROS>
-hipError_t lattice_batch_one(const JetSlot *jets, int njets, uint32_t total_tiles, hipStream_t s) {
+hipError_t lattice_batch_one(const JetSlot *jets, int njets, uint32_t total_tiles, uint32_t *queue, hipStream_t s) {
     static int resident = 0;  // blocks of this kernel that fit on the device at once
     if (resident == 0) {
         int dev = 0, cus = 0, per_cu = 0;
@@ -33,7 +33,7 @@ hipError_t lattice_batch_one(const JetSlot *jets, int njets, uint32_t total_tile
     }
     const uint32_t grid = total_tiles < uint32_t(resident) ? total_tiles : uint32_t(resident);
     hipLaunchKernelGGL((k_lattice_batch<Q, P, TX, TY, MACROS>), dim3(grid), dim3(TX, TY, 1), 0, s, jets, njets,
-                       total_tiles);
+                       total_tiles, queue);
     return hipGetLastError();
 }
 
@@ -53,13 +53,13 @@ hipError_t launch_lattice(int tile, bool macros, const LatticeArgs &a, hipStream
 
 template <int Q, class P>
 hipError_t launch_lattice_batch(int tile, bool macros, const JetSlot *jets, int njets, uint32_t total_tiles,
-                                hipStream_t s) {
-    if (macros) return detail::lattice_batch_one<Q, P, 64, 4, true>(jets, njets, total_tiles, s);
+                                uint32_t *queue, hipStream_t s) {
+    if (macros) return detail::lattice_batch_one<Q, P, 64, 4, true>(jets, njets, total_tiles, queue, s);
     switch (tile) {
-        case 0: return detail::lattice_batch_one<Q, P, 64, 4, false>(jets, njets, total_tiles, s);
-        case 1: return detail::lattice_batch_one<Q, P, 128, 2, false>(jets, njets, total_tiles, s);
-        case 2: return detail::lattice_batch_one<Q, P, 64, 2, false>(jets, njets, total_tiles, s);
-        case 3: return detail::lattice_batch_one<Q, P, 128, 4, false>(jets, njets, total_tiles, s);
+        case 0: return detail::lattice_batch_one<Q, P, 64, 4, false>(jets, njets, total_tiles, queue, s);
+        case 1: return detail::lattice_batch_one<Q, P, 128, 2, false>(jets, njets, total_tiles, queue, s);
+        case 2: return detail::lattice_batch_one<Q, P, 64, 2, false>(jets, njets, total_tiles, queue, s);
+        case 3: return detail::lattice_batch_one<Q, P, 128, 4, false>(jets, njets, total_tiles, queue, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -69,5 +69,6 @@ hipError_t launch_lattice_batch(int tile, bool macros, const JetSlot *jets, int 
 #define SSMD_INSTANTIATE_LATTICE(Q, PT)                                                                         \
     namespace ssmd {                                                                                            \
     template hipError_t launch_lattice<Q, PT>(int, bool, const LatticeArgs &, hipStream_t);                     \
-    template hipError_t launch_lattice_batch<Q, PT>(int, bool, const JetSlot *, int, uint32_t, hipStream_t);    \
+    template hipError_t launch_lattice_batch<Q, PT>(int, bool, const JetSlot *, int, uint32_t, uint32_t *,       \
+                                                    hipStream_t);                                                 \
     }

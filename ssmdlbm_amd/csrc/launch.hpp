@@ -39,10 +39,11 @@ inline int auto_lz(int nx, int ny, int nz, int tile) {
 template <int Q, class P>
 hipError_t launch_lattice(int tile, bool macros, const LatticeArgs &a, hipStream_t s);
 
-// Ensemble: persistent kernel over all members' tiles; grid = resident blocks.
+// Ensemble: persistent kernel over all members' tiles; grid = resident blocks;
+// `queue`: two zeroed words of device memory (the kernel leaves them zeroed).
 template <int Q, class P>
 hipError_t launch_lattice_batch(int tile, bool macros, const JetSlot *jets, int njets, uint32_t total_tiles,
-                                hipStream_t s);
+                                uint32_t *queue, hipStream_t s);
 
 // LEARNED closure pass: tau_t = max(floor, scale * network(feat)) over n
 // cells (closure.hip).

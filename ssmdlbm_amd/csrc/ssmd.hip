@@ -714,6 +714,7 @@ struct ssmd_ensemble {
     uint32_t total_tiles = 0, phase_blocks = 0, face_blocks[4] = {0, 0, 0, 0};
     int parity = 0;
     uint64_t *counter = nullptr;
+    uint32_t *queue = nullptr;  // tile queue of the persistent lattice kernel (2 words, zero between launches)
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
@@ -781,7 +782,7 @@ int ensemble_launch_step(ssmd_ensemble *e, const JetSlot *tab, int tile, uint32_
     const bool macros = (flags & SSMD_STEP_MACROS) != 0;
     if (macros && tile != 0) return SSMD_EINVAL;  // tables are laid out for the tile in use
     SSMD_TRY(dispatch(p0, [&]<int Q, class P>() {
-        return herr(launch_lattice_batch<Q, P>(tile, macros, tab, e->n, total_tiles, s));
+        return herr(launch_lattice_batch<Q, P>(tile, macros, tab, e->n, total_tiles, e->queue, s));
     }));
     return dispatch(p0, [&]<int Q, class P>() {
         for (int i = 0; i < e->nk; ++i) {
@@ -836,6 +837,8 @@ int ssmd_ensemble_create(const ssmd_params *params, const ssmd_fields *fields, i
     for (int par = 0; par < 2 && rc == 0; ++par)
         rc = herr(hipMalloc(reinterpret_cast<void **>(&e->table[par]), sizeof(JetSlot) * size_t(n)));
     if (rc == 0) rc = herr(hipMalloc(reinterpret_cast<void **>(&e->counter), sizeof(uint64_t)));
+    if (rc == 0) rc = herr(hipMalloc(reinterpret_cast<void **>(&e->queue), 2 * sizeof(uint32_t)));
+    if (rc == 0) rc = herr(hipMemset(e->queue, 0, 2 * sizeof(uint32_t)));
     if (rc == 0) rc = ensemble_tables(e, e->tile, e->table, &e->total_tiles);
     if (rc != 0) {
         ssmd_ensemble_destroy(e);
@@ -922,7 +925,7 @@ int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *be
         float ms = 0.f;
         rc = time_launches(s, reps, [&]() {
             return dispatch(p0, [&]<int Q, class P>() {
-                return herr(launch_lattice_batch<Q, P>(t, false, tmp[e->parity], e->n, total, s));
+                return herr(launch_lattice_batch<Q, P>(t, false, tmp[e->parity], e->n, total, e->queue, s));
             });
         }, &ms);
         if (rc == 0 && ms < best) {
@@ -954,6 +957,7 @@ int ssmd_ensemble_destroy(ssmd_ensemble *e) {
     for (int par = 0; par < 2; ++par)
         if (e->table[par]) (void)hipFree(e->table[par]);
     if (e->counter) (void)hipFree(e->counter);
+    if (e->queue) (void)hipFree(e->queue);
     delete e;
     return SSMD_OK;
 }
