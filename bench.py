@@ -318,13 +318,14 @@ def main() -> int:
     forms = None
     if not args.no_forms:
         forms = {}
-        for c in (cases.JET_D3Q19_FP32_LARGE, cases.JET_D3Q19_FP16):
+        for c in (cases.JET_D3Q19_FP32_LARGE, cases.JET_D3Q19_FP16, cases.JET_D3Q19_BF16):
             m = measure(c, args.form_steps, 50, min(args.kernel_steps, 100), tune=not args.no_tune)
             forms[c.name] = {k: round(v, 4) if isinstance(v, float) else v for k, v in m.items()}
             log(f"form {c.name}: {m}")
         a = forms[cases.JET_D3Q19_FP16.name]["mlups"]
         b = forms[cases.JET_D3Q19_FP32_LARGE.name]["mlups"]
         forms["fp16_over_fp32_mlups"] = round(a / b, 4)
+        forms["bf16_over_fp32_mlups"] = round(forms[cases.JET_D3Q19_BF16.name]["mlups"] / b, 4)
         forms["ensemble_32_jets_d3q19"] = measure_ensemble(args.form_steps, 50)
         log(f"ensemble: {forms['ensemble_32_jets_d3q19']}")
         forms["reference_default_case"] = measure_reference_cases(max(args.form_steps, 500), 200)

@@ -112,8 +112,9 @@ JET_D3Q27 = JetCase("jet_d3q27_128x128x512", 128, 128, 512, q=27, sigma=0.05, di
 JET_D3Q19_FP16 = JetCase("jet_d3q19_fp16_256x256x1024", 256, 256, 1024, q=19, sigma=0.05, diameter=24,
                          u=0.04, seed=3, pop_format=abi.POP_F16)
 JET_D3Q19_FP32_LARGE = JET_D3Q19_FP16.with_(name="jet_d3q19_fp32_256x256x1024", pop_format=abi.POP_F32)
+JET_D3Q19_BF16 = JET_D3Q19_FP16.with_(name="jet_d3q19_bf16_256x256x1024", pop_format=abi.POP_BF16)
 
-WORKLOADS = {c.name: c for c in (JET_D3Q19_SMALL, JET_D3Q27, JET_D3Q19_FP16, JET_D3Q19_FP32_LARGE)}
+WORKLOADS = {c.name: c for c in (JET_D3Q19_SMALL, JET_D3Q27, JET_D3Q19_FP16, JET_D3Q19_FP32_LARGE, JET_D3Q19_BF16)}
 
 
 def closed_box(n: int, q: int = 19, tau: float = 0.8, sigma: float = 0.01, gamma: float = 0.3,

@@ -70,12 +70,9 @@ struct PopF16 {
 struct PopBF16 {
     using T = uint16_t;
     static __device__ __forceinline__ float load(T v) { return __builtin_bit_cast(float, uint32_t(v) << 16); }
-    static __device__ __forceinline__ T store(float v) {
-        uint32_t x = __builtin_bit_cast(uint32_t, v);
-        if ((x & 0x7FFFFFFFu) > 0x7F800000u) return uint16_t((x >> 16) | 0x40u);
-        x += 0x7FFFu + ((x >> 16) & 1u);
-        return uint16_t(x >> 16);
-    }
+    // gfx950 converts with round-to-nearest-even in one instruction
+    // (v_cvt_pk_bf16_f32).
+    static __device__ __forceinline__ T store(float v) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v)); }
 };
 
 // --------------------------------------------------------------------------

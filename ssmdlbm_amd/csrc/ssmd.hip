@@ -105,7 +105,6 @@ bool fields_ready(const ssmd_params &p, const ssmd_fields *d) {
 }
 
 int tile_of(const ssmd_params &p) { return (p.tile >= 0 && p.tile < kNumTiles) ? p.tile : 0; }
-int lz_of(const ssmd_params &p) { return p.lz > 0 ? p.lz : auto_lz(int(p.nx), int(p.ny), int(p.nz), tile_of(p)); }
 
 // Calls fn.template operator()<Q, P>() for the lattice and storage format.
 template <class Fn>
