@@ -286,18 +286,23 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         if (y == k.ny - 2) ysm = -sh;
     }
 
-    // phi plane p: element e of the (TX+4)(TY+4) halo plane
-    auto phi_at = [&](int e, int p) -> float {
-        const int gx = x0 + e % HX - 2, gy = y0 + e / HX - 2;
-        const bool ii = p >= 1 && p <= k.nz - 2 && gx >= 1 && gx <= k.nx - 2 && gy >= 1 && gy <= k.ny - 2;
-        return ii ? phi[cell_id(k, gx, gy, p)] : 0.f;
-    };
-    auto fetch_plane = [&](int p, float (&r)[EPT]) {
+    // phi plane p: element e = tid + q NT of the (TX+4)(TY+4) halo plane.
+    // Its in-plane byte offset is the same on every plane; ~0u marks an
+    // element outside the halo plane or on the ghost layer (read as 0).
+    uint32_t hoff[EPT];
 #pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = tid + q * NT;
-            r[q] = e < HP ? phi_at(e, p) : 0.f;
-        }
+    for (int q = 0; q < EPT; ++q) {
+        const int e = tid + q * NT;
+        const int gx = x0 + e % HX - 2, gy = y0 + e / HX - 2;
+        const bool ii = e < HP && gx >= 1 && gx <= k.nx - 2 && gy >= 1 && gy <= k.ny - 2;
+        hoff[q] = ii ? (uint32_t(gx) + uint32_t(k.nx) * uint32_t(gy)) * 4u : ~0u;
+    }
+    auto fetch_plane = [&](int p, float (&r)[EPT]) {
+        const bool plane = p >= 1 && p <= k.nz - 2;
+        const char *base = reinterpret_cast<const char *>(phi + size_t(p) * k.sxy);
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+            r[q] = plane && hoff[q] != ~0u ? *reinterpret_cast<const float *>(base + hoff[q]) : 0.f;
     };
     auto store_plane = [&](int p, const float (&r)[EPT]) {
         const int sl = (p + 5) % 5;
