@@ -231,6 +231,10 @@ __device__ __forceinline__ float closure_tau(const KParams &k, const Macro &mac,
 // waves per SIMD) keep one.
 template <int Q, class P> constexpr int pull_depth() { return Q == 27 && sizeof(typename P::T) == 2 ? 2 : 1; }
 
+// Bytes of global memory, for pointers the compiler must not lose the
+// address space of when they pass through an opaque register copy.
+using GChar = __attribute__((address_space(1))) char;
+
 // One tile: cells x0..x0+TX-1, y0..y0+TY-1, planes zb..ze-1 (threadIdx = (tx, ty)).
 // TX is a multiple of 64, so a wavefront covers part of one x row: y is
 // uniform across a wavefront and z across the block.
@@ -351,8 +355,6 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
     // recomputed on the scalar unit instead of being held (and spilled)
     // across the loop.
     auto pull = [&](int zz, PT (&f)[Q]) {
-        size_t N = N0;
-        asm volatile("" : "+s"(N));
         const uint32_t c = cell_id(k, x, y, zz);
         long zsp = 0, zsm = 0;
         if (k.wrap_z) {
@@ -360,21 +362,32 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             if (zz == 1) zsp = sh;
             if (zz == k.nz - 2) zsm = -sh;
         }
+        // uniform byte steps from a population plane to the source row of
+        // a c_y = +-1 or c_z = +-1 population, periodic wraps included
+        constexpr long B = long(sizeof(PT));
+        const long yp = (ysp - long(k.nx)) * B, ym = (ysm + long(k.nx)) * B;
+        const long zp = (zsp - long(sxy)) * B, zm = (zsm + long(sxy)) * B;
         // uniform base per direction + one of three shared 32-bit byte offsets
         const uint32_t o0 = c * uint32_t(sizeof(PT));
         const uint32_t op = uint32_t(int(c) + xsp) * uint32_t(sizeof(PT));  // c_x > 0 sources
         const uint32_t om = uint32_t(int(c) + xsm) * uint32_t(sizeof(PT));  // c_x < 0 sources
+        // plane i's base is the previous one's plus N: one 64-bit scalar add
+        // per direction, kept from being folded back into i * N
+        const GChar *plane = reinterpret_cast<const GChar *>(reinterpret_cast<uintptr_t>(fin));
+        const size_t NB = size_t(N0) * sizeof(PT);
         sfor<0, Q>([&](auto I) {
             constexpr int i = I;
             constexpr Dir d = kDir[i];
-            long shift = long(i) * long(N) - long(d.x) - long(d.y) * k.nx - long(d.z) * long(sxy);
-            if constexpr (d.y > 0) shift += ysp;
-            if constexpr (d.y < 0) shift += ysm;
-            if constexpr (d.z > 0) shift += zsp;
-            if constexpr (d.z < 0) shift += zsm;
-            const char *base = reinterpret_cast<const char *>(fin + shift);
+            if constexpr (i > 0) plane += NB;
+            asm volatile("" : "+s"(plane));
+            long shift = -long(d.x) * B;
+            if constexpr (d.y > 0) shift += yp;
+            if constexpr (d.y < 0) shift += ym;
+            if constexpr (d.z > 0) shift += zp;
+            if constexpr (d.z < 0) shift += zm;
+            const GChar *base = plane + shift;
             const uint32_t off = d.x > 0 ? op : (d.x < 0 ? om : o0);
-            const PT *src = reinterpret_cast<const PT *>(base + off);
+            const auto *src = reinterpret_cast<const __attribute__((address_space(1))) PT *>(base + off);
             // D3Q27 reads faster with non-temporal loads, D3Q19 slower (measured)
             if constexpr (Q == 27)
                 f[i] = __builtin_nontemporal_load(src);
@@ -514,20 +527,27 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         const float omco = 1.0f - fast_rcp(k.closure == SSMD_CLOSURE_NONE
                                                ? 0.5f + 3.0f * nu
                                                : closure_tau(k, mac, m, c, ph, nc.w, 0.5f + 3.0f * nu, active));
-        // uniform plane base + shared 32-bit byte offset for every store;
-        // non-temporal: the next step reads these from HBM, nothing here does
-        size_t Ns = N0;
-        asm volatile("" : "+s"(Ns));
+        // uniform plane base (the previous one's plus N, as in the pull) +
+        // shared 32-bit byte offset for every store; non-temporal: the next
+        // step reads these from HBM, nothing here does
         const uint32_t so = c * uint32_t(sizeof(PT));
+        GChar *splane = reinterpret_cast<GChar *>(reinterpret_cast<uintptr_t>(fout));
+        const size_t SB = size_t(N0) * sizeof(PT);
         collide<Q>(m, omco, kx, ky, kz, [&](int i, float v) {
-            PT *dst = reinterpret_cast<PT *>(reinterpret_cast<char *>(fout + size_t(i) * Ns) + so);
-            __builtin_nontemporal_store(P::store(active ? v : 0.f), dst);
+            if (i > 0) splane += SB;
+            asm volatile("" : "+s"(splane));
+            __builtin_nontemporal_store(P::store(active ? v : 0.f),
+                                        reinterpret_cast<__attribute__((address_space(1))) PT *>(splane + so));
         });
         const float sharp = k.gamma * ph * (1.0f - ph);
         const uint32_t go = c * 4u;
+        GChar *gplane = reinterpret_cast<GChar *>(reinterpret_cast<uintptr_t>(gout));
+        const size_t GB = size_t(N0) * 4u;
         phase_moments(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) {
-            float *dst = reinterpret_cast<float *>(reinterpret_cast<char *>(gout + size_t(kk) * Ns) + go);
-            __builtin_nontemporal_store(active ? v : 0.f, dst);
+            if (kk > 0) gplane += GB;
+            asm volatile("" : "+s"(gplane));
+            __builtin_nontemporal_store(active ? v : 0.f,
+                                        reinterpret_cast<__attribute__((address_space(1))) float *>(gplane + go));
         });
 
         // rho, u, Pi on the planes the inlet / outflow kernels read them from
