@@ -54,6 +54,23 @@ struct Macro {
     uint16_t *feat;  // LEARNED closure: 8 bf16 network inputs per cell (written)
 };
 
+// The lattice kernel reads its Macro where it is stored (the kernel argument
+// segment, or a jet table), through a pointer made opaque at each use: the
+// compiler then loads the few pointers a boundary plane needs when it needs
+// them instead of holding all 15 in scalar registers across the z loop,
+// which would spill other uniform values.
+using MacroRef = const __attribute__((address_space(4))) Macro *;
+__device__ __forceinline__ MacroRef macro_at(MacroRef m) {
+    asm volatile("" : "+s"(m));
+    return m;
+}
+// A Macro read directly, held in registers where the compiler chooses (the
+// ensemble kernel's jet table: reading it on demand there measured 18% slower).
+__device__ __forceinline__ const Macro *macro_at(const Macro *m) { return m; }
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T *global_ptr(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 __device__ __forceinline__ uint32_t cell_id(const KParams &k, int x, int y, int z) {
     return uint32_t(x) + uint32_t(k.nx) * (uint32_t(y) + uint32_t(k.ny) * uint32_t(z));
 }
@@ -205,7 +222,8 @@ __device__ __forceinline__ uint32_t bf16_pair(float a, float b) {
     const __bf16 lo = static_cast<__bf16>(a), hi = static_cast<__bf16>(b);
     return uint32_t(__builtin_bit_cast(uint16_t, lo)) | (uint32_t(__builtin_bit_cast(uint16_t, hi)) << 16);
 }
-__device__ __forceinline__ float closure_tau(const KParams &k, const Macro &mac, const Cell &m, uint32_t c, float ph,
+template <class MP>
+__device__ __forceinline__ float closure_tau(const KParams &k, MP macp, const Cell &m, uint32_t c, float ph,
                                              float grad, float tau0, bool write_inputs) {
     if (k.closure == SSMD_CLOSURE_SMAGORINSKY) {
         const float q = m.pxx * m.pxx + m.pyy * m.pyy + m.pzz * m.pzz +
@@ -213,14 +231,15 @@ __device__ __forceinline__ float closure_tau(const KParams &k, const Macro &mac,
         return 0.5f * (tau0 + __builtin_amdgcn_sqrtf(tau0 * tau0 + k.smag * __builtin_amdgcn_sqrtf(2.0f * q) *
                                                                          __builtin_amdgcn_rcpf(m.rho)));
     }
-    const float tau = tau0 + mac.tau_t[c];
+    const auto mac = macro_at(macp);
+    const float tau = tau0 + global_ptr(mac->tau_t)[c];
     const float s = -1.5f * k.strain_scale * __builtin_amdgcn_rcpf(m.rho * tau);
     uint4 w;
     w.x = bf16_pair(m.pxx * s, m.pyy * s);
     w.y = bf16_pair(m.pzz * s, m.pxy * s);
     w.z = bf16_pair(m.pxz * s, m.pyz * s);
     w.w = bf16_pair(ph, grad * k.grad_scale);
-    if (write_inputs) *reinterpret_cast<uint4 *>(mac.feat + size_t(c) * 8) = w;
+    if (write_inputs) *reinterpret_cast<uint4 *>(mac->feat + size_t(c) * 8) = w;
     return tau;
 }
 
@@ -238,10 +257,10 @@ using GChar = __attribute__((address_space(1))) char;
 // One tile: cells x0..x0+TX-1, y0..y0+TY-1, planes zb..ze-1 (threadIdx = (tx, ty)).
 // TX is a multiple of 64, so a wavefront covers part of one x row: y is
 // uniform across a wavefront and z across the block.
-template <int Q, class P, int TX, int TY, bool MACROS>
+template <int Q, class P, int TX, int TY, bool MACROS, class MP>
 __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P::T *__restrict__ fin,
                                              typename P::T *__restrict__ fout, float *__restrict__ gout,
-                                             const float *__restrict__ phi, const Macro &mac, int x0, int y0,
+                                             const float *__restrict__ phi, MP macp, int x0, int y0,
                                              int zb, int ze, LatticeSmem<TX, TY> &sm) {
     static_assert(TX % 64 == 0, "a wavefront must not span two rows");
     using S = LatticeSmem<TX, TY>;
@@ -526,7 +545,7 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         const float nu = __builtin_fmaf(ph, k.visc_diff, k.visc_water);
         const float omco = 1.0f - fast_rcp(k.closure == SSMD_CLOSURE_NONE
                                                ? 0.5f + 3.0f * nu
-                                               : closure_tau(k, mac, m, c, ph, nc.w, 0.5f + 3.0f * nu, active));
+                                               : closure_tau(k, macp, m, c, ph, nc.w, 0.5f + 3.0f * nu, active));
         // uniform plane base (the previous one's plus N, as in the pull) +
         // shared 32-bit byte offset for every store; non-temporal: the next
         // step reads these from HBM, nothing here does
@@ -557,21 +576,22 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
                               ((k.bc_planes & 4) && fz_lo) || ((k.bc_planes & 8) && fz_hi);
         if (!active) return;  // ghost-column lanes write populations only
         if (MACROS || bc_plane) {
-            mac.rho[c] = m.rho;
-            mac.ux[c] = m.ux;
-            mac.uy[c] = m.uy;
-            mac.uz[c] = m.uz;
-            mac.pxx[c] = m.pxx;
-            mac.pyy[c] = m.pyy;
-            mac.pzz[c] = m.pzz;
-            mac.pxy[c] = m.pxy;
-            mac.pxz[c] = m.pxz;
-            mac.pyz[c] = m.pyz;
-        }
-        if (MACROS) {
-            mac.fsx[c] = kx;
-            mac.fsy[c] = ky;
-            mac.fsz[c] = kz;
+            const auto mac = macro_at(macp);
+            global_ptr(mac->rho)[c] = m.rho;
+            global_ptr(mac->ux)[c] = m.ux;
+            global_ptr(mac->uy)[c] = m.uy;
+            global_ptr(mac->uz)[c] = m.uz;
+            global_ptr(mac->pxx)[c] = m.pxx;
+            global_ptr(mac->pyy)[c] = m.pyy;
+            global_ptr(mac->pzz)[c] = m.pzz;
+            global_ptr(mac->pxy)[c] = m.pxy;
+            global_ptr(mac->pxz)[c] = m.pxz;
+            global_ptr(mac->pyz)[c] = m.pyz;
+            if (MACROS) {
+                global_ptr(mac->fsx)[c] = kx;
+                global_ptr(mac->fsy)[c] = ky;
+                global_ptr(mac->fsz)[c] = kz;
+            }
         }
     };
     if constexpr (D == 2) {
@@ -584,14 +604,28 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
     }
 }
 
+// Arguments of one lattice launch, passed as the kernel's only argument so
+// that it sits at offset 0 of the kernel argument segment.
+struct LatticeArgs {
+    KParams k;
+    const void *fin;
+    void *fout;
+    float *gout;
+    const float *phi;
+    Macro mac;
+    int lz;
+};
+
 template <int Q, class P, int TX, int TY, bool MACROS>
-__global__ void __launch_bounds__(TX *TY)
-    k_lattice(KParams k, const typename P::T *__restrict__ fin, typename P::T *__restrict__ fout,
-              float *__restrict__ gout, const float *__restrict__ phi, Macro mac, int lz) {
+__global__ void __launch_bounds__(TX *TY) k_lattice(LatticeArgs a) {
     __shared__ LatticeSmem<TX, TY> sm;
-    const int zb = 1 + int(blockIdx.z) * lz;
-    lattice_tile<Q, P, TX, TY, MACROS>(k, fin, fout, gout, phi, mac, int(blockIdx.x) * TX, int(blockIdx.y) * TY, zb,
-                                       min(zb + lz, k.nz - 1), sm);
+    using PT = typename P::T;
+    const auto *seg = reinterpret_cast<const __attribute__((address_space(4))) LatticeArgs *>(
+        reinterpret_cast<uintptr_t>(__builtin_amdgcn_kernarg_segment_ptr()));
+    const int zb = 1 + int(blockIdx.z) * a.lz;
+    lattice_tile<Q, P, TX, TY, MACROS>(a.k, static_cast<const PT *>(a.fin), static_cast<PT *>(a.fout), a.gout, a.phi,
+                                       &seg->mac, int(blockIdx.x) * TX, int(blockIdx.y) * TY, zb,
+                                       min(zb + a.lz, a.k.nz - 1), sm);
 }
 
 // --------------------------------------------------------------------------
@@ -720,7 +754,8 @@ __global__ void __launch_bounds__(TX *TY) k_lattice_batch(const JetSlot *__restr
         const int bz = int(lt / uint32_t(js.tiles_y));
         const int zb = 1 + bz * js.lz;
         lattice_tile<Q, P, TX, TY, MACROS>(js.k, static_cast<const PT *>(js.f), static_cast<PT *>(js.f_next),
-                                           js.g_next, js.phi, js.mac, bx * TX, by * TY, zb,
+                                           js.g_next, js.phi,
+                                           &js.mac, bx * TX, by * TY, zb,
                                            min(zb + js.lz, js.k.nz - 1), sm);
     }
     if (lead &&

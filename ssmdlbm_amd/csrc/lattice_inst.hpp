@@ -10,11 +10,9 @@ namespace detail {
 
 template <int Q, class P, int TX, int TY, bool MACROS>
 hipError_t lattice_one(const LatticeArgs &a, hipStream_t s) {
-    using PT = typename P::T;
     const int lz = a.lz;
     const dim3 grid((a.k.nx + TX - 1) / TX, (a.k.ny + TY - 1) / TY, (a.k.nz - 2 + lz - 1) / lz);
-    hipLaunchKernelGGL((k_lattice<Q, P, TX, TY, MACROS>), grid, dim3(TX, TY, 1), 0, s, a.k,
-                       static_cast<const PT *>(a.fin), static_cast<PT *>(a.fout), a.gout, a.phi, a.mac, lz);
+    hipLaunchKernelGGL((k_lattice<Q, P, TX, TY, MACROS>), grid, dim3(TX, TY, 1), 0, s, a);
     return hipGetLastError();
 }
 

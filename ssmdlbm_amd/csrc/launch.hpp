@@ -14,16 +14,6 @@ struct TileShape {
 constexpr TileShape kTiles[] = {{64, 4}, {128, 2}, {64, 2}, {128, 4}};
 constexpr int kNumTiles = int(sizeof(kTiles) / sizeof(kTiles[0]));
 
-struct LatticeArgs {
-    KParams k;
-    const void *fin;
-    void *fout;
-    float *gout;
-    const float *phi;
-    Macro mac;
-    int lz;
-};
-
 // z planes per block: enough chunks that the grid holds >= ~2048 blocks on
 // large boxes without letting the 4-plane halo prologue dominate a chunk.
 inline int auto_lz(int nx, int ny, int nz, int tile) {
