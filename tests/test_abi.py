@@ -108,6 +108,7 @@ def test_validation_rejects_bad_parameters():
     assert lib.ssmd_validate(b) == -2
     b.nz = 1020  # the largest cube-like box that fits
     assert lib.ssmd_validate(b) == 0
+
     assert lib.ssmd_bytes(p, 1) == 19 * 4 * p.cells
     b = p.copy()
     b.pop_format = abi.POP_F16
@@ -118,6 +119,25 @@ def test_validation_rejects_bad_parameters():
         b = p.copy()
         b.tau_floor = floor
         assert lib.ssmd_validate(b) == rc, floor
+
+
+def test_tile_shapes():
+    lib = abi.load()
+    shapes = []
+    for t in range(lib.ssmd_tile_count()):
+        tx, ty = C.c_int32(), C.c_int32()
+        assert lib.ssmd_tile_shape(t, C.byref(tx), C.byref(ty)) == 0
+        shapes.append((tx.value, ty.value))
+    assert shapes == [(64, 4), (128, 2), (64, 2), (128, 4)]
+    for case in (cases.JET_D3Q19_SMALL, cases.JET_D3Q27):
+        for t in range(len(shapes)):
+            b = case.params().copy()
+            b.tile = t
+            assert lib.ssmd_validate(b) == 0, (case.q, t)
+    b = cases.JET_D3Q19_SMALL.params().copy()
+    b.tile = len(shapes)
+    assert lib.ssmd_validate(b) == -1
+    assert lib.ssmd_tile_shape(len(shapes), C.byref(C.c_int32()), C.byref(C.c_int32())) == -1
 
 
 def test_error_paths_without_device():

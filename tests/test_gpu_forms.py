@@ -46,6 +46,28 @@ def test_ensemble_members_bitwise_equal_solo(gpu, q, graph):
         np.testing.assert_array_equal(m.pops_source("g"), solo.pops_source("g"))
 
 
+@pytest.mark.parametrize("q", [19, 27])
+def test_tile_shapes_agree(gpu, q):
+    # two x tiles of 128 (one partial), partial y tiles for every shape
+    c = cases.JET_D3Q19_SMALL.with_(nx=136, ny=30, nz=40, q=q, diameter=8)
+    lib = abi.load()
+    res = []
+    for tile in range(lib.ssmd_tile_count()):
+        p = c.params()
+        p.tile = tile
+        sim = Simulation(p)
+        sim.init()
+        sim.step(7)
+        sim.step(1, macros=True)
+        res.append((tile, {name: sim.field(name) for name in MACRO}, sim.pops_source("f")))
+        sim.close()
+    assert len(res) == lib.ssmd_tile_count()
+    for tile, fields, f in res[1:]:
+        for name in MACRO:
+            np.testing.assert_array_equal(fields[name], res[0][1][name], err_msg=f"tile {tile} {name}")
+        np.testing.assert_array_equal(f, res[0][2], err_msg=f"tile {tile} f")
+
+
 def test_ensemble_tile_shapes_agree(gpu):
     members = small_ensemble(n=3)
     res = []
