@@ -169,6 +169,13 @@ __device__ __forceinline__ Cell moments(const float (&f)[Q], float fx, float fy,
 // Emits w_i * (even +- odd) for every direction through `emit(i, value)`.
 // omco = 1 - omega.  With omco and F the reference's values this is
 // feq + omco*fneq + F/2; a boundary reconstruction without force passes F=0.
+// With c_u = 3 c.u, hc = c_u^2/2 - uu (uu = 3/2 u.u) and h2 = c c : Pi - t:
+//     even = drho (1 + hc) + hc + 4.5 omco h2 + 1.5 (c_u c.F - u.F)
+//          = E0 + c_u (rho c_u / 2 + 1.5 c.F) + 4.5 omco h2,
+//       E0 = drho - rho uu - 1.5 u.F;
+//     odd  = rho c_u (1 [+ c_u^2/6 - uu]) + 1.5 c.F [+ 4.5 omco h3],
+// the bracketed terms for D3Q27 (D3Q27.cuh:358-367 for h3).  The per-cell
+// sums are formed once, so a direction pair costs 7-10 operations.
 template <int Q, class Emit>
 __device__ __forceinline__ void collide(const Cell &m, float omco, float fx, float fy, float fz, Emit &&emit) {
     const float ux = m.ux, uy = m.uy, uz = m.uz;
@@ -176,6 +183,12 @@ __device__ __forceinline__ void collide(const Cell &m, float omco, float fx, flo
     const float uF = ux * fx + uy * fy + uz * fz;
     const float t = kCs2 * (m.pxx + m.pyy + m.pzz);
     const float o45 = 4.5f * omco;
+    const float rho = m.rho, hr = 0.5f * m.rho;
+    const float E0 = __builtin_fmaf(-1.5f, uF, __builtin_fmaf(-rho, uu, m.drho));
+    const float u3[3] = {3.0f * ux, 3.0f * uy, 3.0f * uz};
+    const float F15[3] = {1.5f * fx, 1.5f * fy, 1.5f * fz};
+    const float D[3] = {m.pxx - t, m.pyy - t, m.pzz - t};  // c_a c_a - t, one axis
+    const float P2[3] = {2.0f * m.pxy, 2.0f * m.pxz, 2.0f * m.pyz};
     // third-order coefficient combinations (D3Q27.cuh:358-367)
     float axxy = 0.f, axxz = 0.f, axyy = 0.f, ayyz = 0.f, axzz = 0.f, ayzz = 0.f, axyz = 0.f;
     if constexpr (Q == 27) {
@@ -187,29 +200,32 @@ __device__ __forceinline__ void collide(const Cell &m, float omco, float fx, flo
         ayzz = m.pzz * uy + 2.0f * uz * m.pyz;
         axyz = ux * m.pyz + uy * m.pxz + uz * m.pxy;
     }
-    // rest population
-    {
-        constexpr float w0 = weight<Q>(0);
-        const float e = m.drho * (1.0f - uu) - uu - o45 * t - 1.5f * uF;
-        emit(0, w0 * e);
-    }
+    // rest population: even with c = 0
+    emit(0, weight<Q>(0) * __builtin_fmaf(-o45, t, E0));
     sfor<0, (Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * P + 1;
         constexpr Dir c = kDir[i];
         constexpr float w = weight<Q>(i);
-        float cu = 0.f, cF = 0.f, h2 = -t;
-        if constexpr (c.x != 0) { cu += c.x > 0 ? ux : -ux; cF += c.x > 0 ? fx : -fx; h2 += m.pxx; }
-        if constexpr (c.y != 0) { cu += c.y > 0 ? uy : -uy; cF += c.y > 0 ? fy : -fy; h2 += m.pyy; }
-        if constexpr (c.z != 0) { cu += c.z > 0 ? uz : -uz; cF += c.z > 0 ? fz : -fz; h2 += m.pzz; }
-        if constexpr (c.x * c.y != 0) h2 += c.x * c.y > 0 ? 2.0f * m.pxy : -2.0f * m.pxy;
-        if constexpr (c.x * c.z != 0) h2 += c.x * c.z > 0 ? 2.0f * m.pxz : -2.0f * m.pxz;
-        if constexpr (c.y * c.z != 0) h2 += c.y * c.z > 0 ? 2.0f * m.pyz : -2.0f * m.pyz;
-        cu *= 3.0f;
-        const float hc = 0.5f * cu * cu - uu;  // X - 1 with X = 1 - uu + cu^2/2
-        const float even = m.drho * (1.0f + hc) + hc + o45 * h2 + 1.5f * (cu * cF - uF);
-        float odd_u = 1.0f;
-        if constexpr (Q == 27) odd_u += cu * cu * (1.0f / 6.0f) - uu;
-        float odd = m.rho * cu * odd_u + 1.5f * cF;
+        constexpr int cv[3] = {c.x, c.y, c.z};
+        float cu = 0.f, cF = 0.f, h2 = 0.f;
+        bool first = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (cv[a] == 0) continue;
+            const float su = cv[a] > 0 ? u3[a] : -u3[a], sf = cv[a] > 0 ? F15[a] : -F15[a];
+            // h2 = sum over the nonzero axes of (p_aa) - t: the first axis
+            // brings D = p_aa - t, the others p_aa
+            const float pa = a == 0 ? m.pxx : (a == 1 ? m.pyy : m.pzz);
+            cu = first ? su : cu + su;
+            cF = first ? sf : cF + sf;
+            h2 = first ? D[a] : h2 + pa;
+            first = false;
+        }
+        if constexpr (c.x * c.y != 0) h2 = __builtin_fmaf(float(c.x * c.y), P2[0], h2);
+        if constexpr (c.x * c.z != 0) h2 = __builtin_fmaf(float(c.x * c.z), P2[1], h2);
+        if constexpr (c.y * c.z != 0) h2 = __builtin_fmaf(float(c.y * c.z), P2[2], h2);
+        const float even = __builtin_fmaf(cu, __builtin_fmaf(hr, cu, cF), __builtin_fmaf(o45, h2, E0));
+        float odd;
         if constexpr (Q == 27) {
             // 3 * (c_a c_a c_b - cs2 c_b) and 6 c_x c_y c_z, all integers
             constexpr int kxxy = 3 * c.x * c.x * c.y - c.y;
@@ -220,17 +236,21 @@ __device__ __forceinline__ void collide(const Cell &m, float omco, float fx, flo
             constexpr int kyzz = 3 * c.y * c.z * c.z - c.y;
             constexpr int kxyz = 6 * c.x * c.y * c.z;
             float h3 = 0.f;
-            if constexpr (kxxy != 0) h3 += float(kxxy) * axxy;
-            if constexpr (kxxz != 0) h3 += float(kxxz) * axxz;
-            if constexpr (kxyy != 0) h3 += float(kxyy) * axyy;
-            if constexpr (kyyz != 0) h3 += float(kyyz) * ayyz;
-            if constexpr (kxzz != 0) h3 += float(kxzz) * axzz;
-            if constexpr (kyzz != 0) h3 += float(kyzz) * ayzz;
-            if constexpr (kxyz != 0) h3 += float(kxyz) * axyz;
-            odd += o45 * h3;
+            if constexpr (kxxy != 0) h3 = __builtin_fmaf(float(kxxy), axxy, h3);
+            if constexpr (kxxz != 0) h3 = __builtin_fmaf(float(kxxz), axxz, h3);
+            if constexpr (kxyy != 0) h3 = __builtin_fmaf(float(kxyy), axyy, h3);
+            if constexpr (kyyz != 0) h3 = __builtin_fmaf(float(kyyz), ayyz, h3);
+            if constexpr (kxzz != 0) h3 = __builtin_fmaf(float(kxzz), axzz, h3);
+            if constexpr (kyzz != 0) h3 = __builtin_fmaf(float(kyzz), ayzz, h3);
+            if constexpr (kxyz != 0) h3 = __builtin_fmaf(float(kxyz), axyz, h3);
+            const float rc = rho * cu;
+            odd = __builtin_fmaf(o45, h3, __builtin_fmaf(rc, __builtin_fmaf(cu * (1.0f / 6.0f), cu, -uu), rc + cF));
+        } else {
+            odd = __builtin_fmaf(rho, cu, cF);
         }
-        emit(i, w * (even + odd));
-        emit(i + 1, w * (even - odd));
+        const float we = w * even;
+        emit(i, __builtin_fmaf(w, odd, we));
+        emit(i + 1, __builtin_fmaf(-w, odd, we));
     });
 }
 
