@@ -369,10 +369,9 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
     if (zb + 1 < ze) fetch_plane(zb + 3, nxt);  // one plane ahead, in registers
     __syncthreads();  // plane zb-2 (read above) is the first ring slot reused
 
-    // Pull the Q populations arriving at (x, y, zz), as stored.  The plane
-    // stride is made opaque per call so the 2Q uniform plane offsets are
-    // recomputed on the scalar unit instead of being held (and spilled)
-    // across the loop.
+    // Pull the Q populations arriving at (x, y, zz), as stored.  Each pull's
+    // uniform base is computed on the scalar unit right where it is used
+    // (not held, and spilled, across the loop).
     auto pull = [&](int zz, PT (&f)[Q]) {
         const uint32_t c = cell_id(k, x, y, zz);
         long zsp = 0, zsm = 0;
