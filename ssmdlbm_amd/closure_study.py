@@ -531,7 +531,11 @@ def calibrate_main(args) -> int:
     time tau_t = max(floor, 1e-3 (kappa x7 + beta)): Nelder-Mead on (kappa,
     beta) minimising the largest of the three statistic errors of a coarse
     LEARNED run of a training jet against that jet's filtered fine run; the
-    result is scored on the held-out jet of an evaluation report."""
+    result is scored on the held-out jet of an evaluation report.
+
+    With --capillary a third parameter s scales the coarse run's surface
+    tension (sigma_coarse = s sigma): a coarse-grid capillary closure, the
+    lever the interface statistics answer to and tau_t cannot reach."""
     from scipy.optimize import minimize
 
     out = Path(args.out)
@@ -546,33 +550,49 @@ def calibrate_main(args) -> int:
     fine_train = run_fine(g, train, steps, every, stat_from, [], z_range, kmax)
     history = []
 
-    def errors(jet, fine, kappa, beta):
+    def errors(jet, fine, kappa, beta, s=1.0):
         net = Network(interface_network(kappa, beta),
                       scales=Scales(strain_scale=1.0, grad_scale=args.grad_scale, tau_scale=1e-3, tau_floor=floor))
-        r = run_coarse(g, jet, abi.CLOSURE_LEARNED, net, steps, every, stat_from, z_range, kmax, tune=False)
+
+        def edit(p):
+            p.sigma = p.sigma * s
+
+        r = run_coarse(g, jet, abi.CLOSURE_LEARNED, net, steps, every, stat_from, z_range, kmax, tune=False,
+                       edit=edit if s != 1.0 else None)
         e = compare(r, fine, z_range, kmax) if r["finite"] else None
         return e, r
 
     def objective(v):
-        e, r = errors(train, fine_train, float(v[0]), float(v[1]))
+        s = float(v[2]) if args.capillary else 1.0
+        if not 0.5 <= s <= 2.0:  # outside the range the sensitivity study covers
+            return 10.0
+        e, r = errors(train, fine_train, float(v[0]), float(v[1]), s)
         j = max(e.values()) if e else 10.0
-        history.append({"kappa": float(v[0]), "beta": float(v[1]), "errors": e, "objective": j,
+        history.append({"kappa": float(v[0]), "beta": float(v[1]), "sigma_scale": s, "errors": e, "objective": j,
                         "tau_t_mean": r.get("tau_t_mean"), "tau_t_max": r.get("tau_t_max")})
-        log(f"kappa {v[0]:8.3f} beta {v[1]:8.3f}: max error {j:.4f} {e}")
+        log(f"kappa {v[0]:8.3f} beta {v[1]:8.3f} s {s:6.3f}: max error {j:.4f} {e}")
         return j
 
-    res = minimize(objective, x0=np.array([0.0, 0.0]), method="Nelder-Mead",
-                   options={"maxfev": args.evals, "initial_simplex": np.array([[0.0, 0.0], [args.step, 0.0],
-                                                                             [0.0, args.step]])})
+    x0 = [args.kappa0, args.beta0] + ([1.0] if args.capillary else [])
+    simplex = [list(x0)]
+    for i in range(len(x0)):
+        v = list(x0)
+        v[i] += args.sigma_step if i == 2 else args.step
+        simplex.append(v)
+    res = minimize(objective, x0=np.array(x0), method="Nelder-Mead",
+                   options={"maxfev": args.evals, "initial_simplex": np.array(simplex)})
     kappa, beta = float(res.x[0]), float(res.x[1])
+    s_best = float(res.x[2]) if args.capillary else 1.0
     held_fine = ref["filtered_fine"]
-    e_best, _ = errors(held, held_fine, kappa, beta)
+    e_best, _ = errors(held, held_fine, kappa, beta, s_best)
     e_none = ref["errors"]["none"]
     rep = {"train_jet": asdict(train), "held_out_jet": asdict(held), "floor": floor, "grad_scale": args.grad_scale,
-           "history": history, "best": {"kappa": kappa, "beta": beta, "train_objective": float(res.fun)},
+           "history": history,
+           "best": {"kappa": kappa, "beta": beta, "sigma_scale": s_best, "train_objective": float(res.fun)},
            "held_out": {"calibrated": e_best, "none": e_none}}
     (out / "calibrate_report.json").write_text(json.dumps(rep, indent=1))
-    log(f"best kappa {kappa:.3f} beta {beta:.3f}; held-out calibrated {e_best}; none {e_none}")
+    log(f"best kappa {kappa:.3f} beta {beta:.3f} sigma scale {s_best:.3f}; held-out calibrated {e_best}; "
+        f"none {e_none}")
     return 0
 
 
@@ -622,6 +642,11 @@ def main(argv=None) -> int:
     ca.add_argument("--grad-scale", type=float, default=4.0)
     ca.add_argument("--step", type=float, default=10.0, help="initial simplex step (units of 1e-3 in tau)")
     ca.add_argument("--evals", type=int, default=30)
+    ca.add_argument("--kappa0", type=float, default=0.0, help="starting point")
+    ca.add_argument("--beta0", type=float, default=0.0, help="starting point")
+    ca.add_argument("--capillary", action="store_true",
+                    help="also fit s, the coarse surface-tension scale (sigma_coarse = s sigma)")
+    ca.add_argument("--sigma-step", type=float, default=0.25, help="initial simplex step of s")
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("the closure study runs on the GPU", file=sys.stderr)
