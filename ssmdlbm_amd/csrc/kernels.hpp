@@ -536,10 +536,7 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         }
 
         // (5) moments; then the pull of plane z+D; then the collision
-        float f[Q];
-#pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = P::load(fr[i]);
-        const Cell m = moments<Q>(f, kx, ky, kz);
+        const Cell m = moments<Q, P>(fr, kx, ky, kz);
         if (z + D < ze) pull(z + D, fr);
         const float nu = __builtin_fmaf(ph, k.visc_diff, k.visc_water);
         const float omco = 1.0f - fast_rcp(k.closure == SSMD_CLOSURE_NONE

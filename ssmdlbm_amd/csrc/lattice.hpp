@@ -48,10 +48,16 @@ constexpr float kCs2 = float(1.0 / 3.0);
 
 // --------------------------------------------------------------------------
 // Population storage formats (deviation fbar = f - w is what is stored).
+// pair(a, b, one, sum, dif): the FP32 sum and difference of two stored
+// values, exactly rounded (`one` is 1.0f held opaque by the caller).
 struct PopF32 {
     using T = float;
     static __device__ __forceinline__ float load(T v) { return v; }
     static __device__ __forceinline__ T store(float v) { return v; }
+    static __device__ __forceinline__ void pair(T a, T b, float, float &sum, float &dif) {
+        sum = a + b;
+        dif = a - b;
+    }
 };
 struct PopF16 {
     using T = uint16_t;
@@ -66,6 +72,14 @@ struct PopF16 {
         asm volatile("" : "+v"(v));
         return __builtin_bit_cast(uint16_t, _Float16(v));
     }
+    // a * 1 + b rounds exactly as a + b; as an FMA of f16 operands it is one
+    // mixed-precision instruction (v_fma_mix_f32) instead of two conversions
+    // and an add
+    static __device__ __forceinline__ void pair(T a, T b, float one, float &sum, float &dif) {
+        const float x = load(a), y = load(b);
+        sum = __builtin_fmaf(x, one, y);
+        dif = __builtin_fmaf(x, one, -y);
+    }
 };
 struct PopBF16 {
     using T = uint16_t;
@@ -73,6 +87,11 @@ struct PopBF16 {
     // gfx950 converts with round-to-nearest-even in one instruction
     // (v_cvt_pk_bf16_f32).
     static __device__ __forceinline__ T store(float v) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v)); }
+    static __device__ __forceinline__ void pair(T a, T b, float, float &sum, float &dif) {
+        const float x = load(a), y = load(b);
+        sum = x + y;
+        dif = x - y;
+    }
 };
 
 // --------------------------------------------------------------------------
@@ -121,17 +140,19 @@ struct Cell {
     float pxx, pyy, pzz, pxy, pxz, pyz;
 };
 
-template <int Q>
-__device__ __forceinline__ Cell moments(const float (&f)[Q], float fx, float fy, float fz) {
+template <int Q, class P>
+__device__ __forceinline__ Cell moments(const typename P::T (&f)[Q], float fx, float fy, float fz) {
     Cell m;
-    float dr = f[0];
+    float one = 1.0f;
+    asm volatile("" : "+v"(one));  // not folded: P::pair multiplies by it
+    float dr = P::load(f[0]);
     float jx = 0.f, jy = 0.f, jz = 0.f;
     float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
-    sfor<0, (Q - 1) / 2>([&](auto P) {
-        constexpr int i = 2 * P + 1;
+    sfor<0, (Q - 1) / 2>([&](auto Pp) {
+        constexpr int i = 2 * Pp + 1;
         constexpr Dir c = kDir[i];
-        const float a = f[i], b = f[i + 1];
-        const float sum = a + b, dif = a - b;
+        float sum, dif;
+        P::pair(f[i], f[i + 1], one, sum, dif);
         dr += sum;
         if constexpr (c.x != 0) { jx += c.x > 0 ? dif : -dif; sxx += sum; }
         if constexpr (c.y != 0) { jy += c.y > 0 ? dif : -dif; syy += sum; }
