@@ -46,26 +46,31 @@ def test_ensemble_members_bitwise_equal_solo(gpu, q, graph):
         np.testing.assert_array_equal(m.pops_source("g"), solo.pops_source("g"))
 
 
-@pytest.mark.parametrize("q", [19, 27])
-def test_tile_shapes_agree(gpu, q):
-    # two x tiles of 128 (one partial), partial y tiles for every shape
-    c = cases.JET_D3Q19_SMALL.with_(nx=136, ny=30, nz=40, q=q, diameter=8)
+@pytest.mark.parametrize("q,pop", [(19, abi.POP_F32), (27, abi.POP_F32), (27, abi.POP_F16)])
+def test_tile_shapes_agree(gpu, q, pop):
+    # two x tiles of 128 (one partial), partial y tiles for every shape; for
+    # the 16-bit D3Q27 kernels (two planes of pulls in flight, the z loop
+    # unrolled by two) also z chunks of odd length and a partial last chunk
+    c = cases.JET_D3Q19_SMALL.with_(nx=136, ny=30, nz=40, q=q, diameter=8, pop_format=pop)
     lib = abi.load()
     res = []
-    for tile in range(lib.ssmd_tile_count()):
+    runs = [(t, 0) for t in range(lib.ssmd_tile_count())]
+    if pop != abi.POP_F32:
+        runs += [(3, 5), (0, 7), (3, 1)]
+    for tile, lz in runs:
         p = c.params()
-        p.tile = tile
+        p.tile, p.lz = tile, lz
         sim = Simulation(p)
         sim.init()
         sim.step(7)
         sim.step(1, macros=True)
-        res.append((tile, {name: sim.field(name) for name in MACRO}, sim.pops_source("f")))
+        res.append((f"tile {tile} lz {lz}", {name: sim.field(name) for name in MACRO}, sim.pops_source("f")))
         sim.close()
-    assert len(res) == lib.ssmd_tile_count()
-    for tile, fields, f in res[1:]:
+    assert len(res) == len(runs)
+    for what, fields, f in res[1:]:
         for name in MACRO:
-            np.testing.assert_array_equal(fields[name], res[0][1][name], err_msg=f"tile {tile} {name}")
-        np.testing.assert_array_equal(f, res[0][2], err_msg=f"tile {tile} f")
+            np.testing.assert_array_equal(fields[name], res[0][1][name], err_msg=f"{what} {name}")
+        np.testing.assert_array_equal(f, res[0][2], err_msg=f"{what} f")
 
 
 def test_ensemble_tile_shapes_agree(gpu):
