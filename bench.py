@@ -28,6 +28,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix peak
+# closure network 8 -> 64 -> 64 -> 64 -> 1: 2 flop per multiply-add
+MLP_FLOP_PER_CELL = 2 * (8 * 64 + 64 * 64 + 64 * 64 + 64 * 1)
 METRIC = "MLUPS + % HBM roofline, D3Q19/27+D3Q7 jet; FP16-storage vs FP32"
 
 
@@ -98,6 +101,64 @@ def measure(case, steps: int, warmup: int, kernel_steps: int, tune: bool = True)
         "step_frac_own_bytes": mlups * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9),
         "state_finite": finite,
         "max_abs_uz": max_uz,
+    }
+
+
+def measure_closure(case, tile, steps: int, warmup: int, kernel_steps: int):
+    """The workload with the LEARNED closure and the trained network
+    (ssmdlbm_amd/data/closure_jet_d3q27): MLUPS of the whole step, and the
+    network pass over every cell (ssmd_closure_update: 8 -> 64 -> 64 -> 64 -> 1
+    on bf16 MFMA) timed alone with events on the simulation's stream."""
+    import torch
+
+    from ssmdlbm_amd import abi
+    from ssmdlbm_amd.closure import Network
+    from ssmdlbm_amd.solver import Simulation
+
+    net = Network.load(ROOT / "ssmdlbm_amd" / "data" / "closure_jet_d3q27")
+    p = case.params()
+    p.closure = abi.CLOSURE_LEARNED
+    p.tile, p.lz = tile
+    net.scales.apply(p)
+    sim = Simulation(p, "cuda:0")
+    net.load_into(sim)
+    sim.init()
+    sim.graph_run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.graph_run(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stream = sim.stream
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(kernel_steps)]
+    for e0, e1 in ev:  # the same features each time: tau_t is rewritten with the same values
+        e0.record(stream)
+        sim.closure_update()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    k_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / len(ev)
+    sim.step(1, macros=True)
+    sim.synchronize()
+    finite = bool(torch.isfinite(sim.t["uz"]).all())
+    tau_t = sim.t["tau_t"]
+    tau_mean, tau_max = float(tau_t.mean()), float(tau_t.max())
+    sim.close()
+    del sim, net
+    torch.cuda.empty_cache()
+    cells = p.cells
+    flop = cells * MLP_FLOP_PER_CELL
+    return {
+        "workload": case.name + "_learned_closure",
+        "mlups": round(cells * steps / wall / 1e6, 1),
+        "ms_per_step": round(wall / steps * 1e3, 5),
+        "network_kernel_ms": round(k_ms, 5),
+        "network_flop_per_cell": MLP_FLOP_PER_CELL,
+        "network_tflops": round(flop / (k_ms * 1e-3) / 1e12, 1),
+        "network_frac_of_bf16_mfma_peak": round(flop / (k_ms * 1e-3) / 1e12 / BF16_MFMA_PEAK_TFLOPS, 4),
+        "network_hbm_gbs": round(cells * 20 / (k_ms * 1e-3) / 1e9, 1),
+        "tau_t_mean": tau_mean,
+        "tau_t_max": tau_max,
+        "state_finite": finite,
     }
 
 
@@ -330,6 +391,11 @@ def main() -> int:
         log(f"ensemble: {forms['ensemble_32_jets_d3q19']}")
         forms["reference_default_case"] = measure_reference_cases(max(args.form_steps, 500), 200)
         log(f"reference default case: {forms['reference_default_case']}")
+        forms["learned_closure"] = measure_closure(case, main_m["tile"], args.form_steps, 50,
+                                                   min(args.kernel_steps, 100))
+        forms["learned_closure"]["over_no_closure_mlups"] = round(forms["learned_closure"]["mlups"] /
+                                                                  main_m["mlups"], 4)
+        log(f"learned closure: {forms['learned_closure']}")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -141,19 +141,50 @@ __device__ __forceinline__ bf16x8 load_x(const uint16_t *__restrict__ x, uint32_
 }
 
 // --------------------------------------------------------------------------
-// Inference: each wavefront keeps the whole network in registers and walks
-// 32-cell tiles.  closure != 0 writes max(floor, scale * y) (the eddy
-// relaxation time), else y.
-__global__ void __launch_bounds__(256) k_forward(const uint8_t *__restrict__ packed, const uint16_t *__restrict__ x,
+// Inference: each wavefront keeps the weight operands in registers (72
+// VGPRs) and walks 32-cell tiles.  The biases and W4 live once per block in
+// LDS in unit order (1 KB): a lane's accumulator image of one block mb is
+// four runs of four consecutive units, mb*32 + j*8 + (l/32)*4 + 0..3, one
+// broadcast 16-byte LDS read each, so they initialize the accumulators
+// without holding 128 VGPRs of replicated values (3 waves per SIMD instead
+// of 1).  closure != 0 writes max(floor, scale * y) (the eddy relaxation
+// time), else y.
+__device__ __forceinline__ f32x16 unit_image(const float *v, int mb, int lane) {
+    const int base = mb * 32 + (lane / 32) * 4;
+    f32x16 r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float4 q = *reinterpret_cast<const float4 *>(v + base + j * 8);
+        r[4 * j + 0] = q.x;
+        r[4 * j + 1] = q.y;
+        r[4 * j + 2] = q.z;
+        r[4 * j + 3] = q.w;
+    }
+    return r;
+}
+
+// (at least 3 waves per SIMD: keeps the compiler from parking accumulators in
+// AGPRs beyond the 168 registers that allows; the kernel needs 126)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_forward(const uint8_t *__restrict__ packed, const uint16_t *__restrict__ x,
                                                  float *__restrict__ y, uint32_t n, float scale, float floor,
                                                  int closure) {
+    // b1, b2, b3, W4 in unit order, read back from the accumulator images:
+    // lane 0 holds units mb*32 + j*8 + i, lane 32 units mb*32 + j*8 + 4 + i
+    // (i = 0..3) in register 4j + i
+    __shared__ __attribute__((aligned(16))) float units[4][H];
+    {
+        const float *img = reinterpret_cast<const float *>(packed + off::B1);
+        const int t = int(threadIdx.x), a = t / H, u = t % H;  // 256 threads = 4 x 64 entries
+        const int mb = u / 32, rem = u % 32, j = rem / 8, half = (rem % 8) / 4, i = rem % 4;
+        units[a][u] = img[(size_t(a * 2 + mb) * 64 + half * 32) * 16 + 4 * j + i];
+    }
+    __syncthreads();
     const int lane = int(threadIdx.x) & 63;
     const bf16x8 *A1 = reinterpret_cast<const bf16x8 *>(packed + off::A1);
     const bf16x8 *A2 = reinterpret_cast<const bf16x8 *>(packed + off::A2);
     const bf16x8 *A3 = reinterpret_cast<const bf16x8 *>(packed + off::A3);
-    const f32x16 *Bs = reinterpret_cast<const f32x16 *>(packed + off::B1);
     bf16x8 a1[2], a2[2][4], a3[2][4];
-    f32x16 b1[2], b2[2], b3[2], w4[2];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
         a1[mb] = A1[mb * 64 + lane];
@@ -162,10 +193,6 @@ __global__ void __launch_bounds__(256) k_forward(const uint8_t *__restrict__ pac
             a2[mb][kc] = A2[(mb * 4 + kc) * 64 + lane];
             a3[mb][kc] = A3[(mb * 4 + kc) * 64 + lane];
         }
-        b1[mb] = Bs[(0 * 2 + mb) * 64 + lane];
-        b2[mb] = Bs[(1 * 2 + mb) * 64 + lane];
-        b3[mb] = Bs[(2 * 2 + mb) * 64 + lane];
-        w4[mb] = Bs[(3 * 2 + mb) * 64 + lane];
     }
     const float b4 = *reinterpret_cast<const float *>(packed + off::B4);
 
@@ -175,17 +202,64 @@ __global__ void __launch_bounds__(256) k_forward(const uint8_t *__restrict__ pac
         const uint32_t cell = t * 32 + uint32_t(lane & 31);
         const bool ok = cell < n;
         const bf16x8 xin = load_x(x, cell, lane < 32 && ok);
+        // an offset the compiler cannot see is 0: the LDS reads below stay in
+        // the loop instead of being hoisted into 128 registers
+        int z0 = 0;
+        asm volatile("" : "+v"(z0));
+        const float *u0 = &units[0][0] + z0;
         f32x16 c[2];
         bf16x8 b[4];
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb) c[mb] = mfma(a1[mb], xin, b1[mb]);
+        for (int mb = 0; mb < 2; ++mb) c[mb] = mfma(a1[mb], xin, unit_image(u0, mb, lane));
         chunks<true>(c, b);
-        layer64([&](int mb, int kc) { return a2[mb][kc]; }, b, b2, c);
+        // each block's bias image is read just before its chain of products
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+            f32x16 acc = unit_image(u0 + H, mb, lane);
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) acc = mfma(a2[mb][kc], b[kc], acc);
+            c[mb] = acc;
+        }
         chunks<true>(c, b);
-        layer64([&](int mb, int kc) { return a3[mb][kc]; }, b, b3, c);
-        const float o = out_dot(c, w4, b4);
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+            f32x16 acc = unit_image(u0 + 2 * H, mb, lane);
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) acc = mfma(a3[mb][kc], b[kc], acc);
+            c[mb] = acc;
+        }
+        // output: relu(c) . W4, four units of W4 at a time from LDS
+        float o = 0.f;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float4 w = *reinterpret_cast<const float4 *>(u0 + 3 * H + mb * 32 + (lane / 32) * 4 + j * 8);
+                o += __builtin_fmaxf(c[mb][4 * j + 0], 0.f) * w.x;
+                o += __builtin_fmaxf(c[mb][4 * j + 1], 0.f) * w.y;
+                o += __builtin_fmaxf(c[mb][4 * j + 2], 0.f) * w.z;
+                o += __builtin_fmaxf(c[mb][4 * j + 3], 0.f) * w.w;
+            }
+        o = o + __shfl_xor(o, 32) + b4;  // the association of out_dot
         if (lane < 32 && ok) y[cell] = closure ? __builtin_fmaxf(scale * o, floor) : o;
     }
+}
+
+// Blocks of k_forward for n cells: at most the blocks resident at once, so
+// each wavefront loads the weight operands once and walks many tiles.
+inline uint32_t forward_blocks(uint32_t n) {
+    static uint32_t resident = 0;
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_forward, 256, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        resident = uint32_t(cus) * uint32_t(per_cu);
+    }
+    const uint32_t tiles = (n + 31) / 32, want = (tiles + 3) / 4;
+    return want < resident ? want : resident;
 }
 
 // --------------------------------------------------------------------------
@@ -626,10 +700,7 @@ int ssmd_mlp_forward(const void *packed, const uint16_t *x, float *y, uint32_t n
     if (!packed || (n && (!x || !y))) return SSMD_EINVAL;
     if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return SSMD_EINVAL;
     if (n == 0) return SSMD_OK;
-    const uint32_t tiles = (n + 31) / 32;
-    uint32_t blocks = (tiles + 3) / 4;
-    if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_forward, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+    hipLaunchKernelGGL(k_forward, dim3(forward_blocks(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
                        static_cast<const uint8_t *>(packed), x, y, n, 1.0f, 0.0f, 0);
     return herr(hipGetLastError());
 }
@@ -729,11 +800,8 @@ namespace ssmd {
 // tau_t = max(floor, scale * network(feat)) over n cells (the LEARNED closure).
 int launch_closure_update(const void *packed, const uint16_t *feat, float *tau_t, uint32_t n, float scale,
                           float floor, hipStream_t s) {
-    const uint32_t tiles = (n + 31) / 32;
-    uint32_t blocks = (tiles + 3) / 4;
-    if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(mlp::k_forward, dim3(blocks), dim3(256), 0, s, static_cast<const uint8_t *>(packed), feat,
-                       tau_t, n, scale, floor, 1);
+    hipLaunchKernelGGL(mlp::k_forward, dim3(mlp::forward_blocks(n)), dim3(256), 0, s,
+                       static_cast<const uint8_t *>(packed), feat, tau_t, n, scale, floor, 1);
     return herr(hipGetLastError());
 }
 }  // namespace ssmd
