@@ -68,10 +68,20 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-// Two floats -> two bf16 (round to nearest even) in one word, low first.
+// Two floats -> two bf16 (round to nearest even) in one word, low first:
+// one two-operand v_cvt_pk_bf16_f32 (two scalar conversions would take two
+// of them and a v_perm_b32).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t pk(float a, float b) {
-    const __bf16 lo = static_cast<__bf16>(a), hi = static_cast<__bf16>(b);
-    return uint32_t(__builtin_bit_cast(uint16_t, lo)) | (uint32_t(__builtin_bit_cast(uint16_t, hi)) << 16);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
+// max(x, 0) of a finite float on its bit pattern: negative floats (sign bit
+// set, -0 included) are negative integers.  One v_max_i32; fmaxf would add a
+// canonicalizing v_max_f32.
+__device__ __forceinline__ float relu(float x) {
+    const int v = __builtin_bit_cast(int, x);
+    return __builtin_bit_cast(float, v > 0 ? v : 0);
 }
 // Same, then ReLU on the pair as 16-bit integers: a bf16 is negative exactly
 // when its sign bit is set, and +0 is the zero pattern.
@@ -130,7 +140,7 @@ __device__ __forceinline__ float out_dot(const f32x16 (&c3)[2], const f32x16 (&w
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) s += __builtin_fmaxf(c3[mb][r], 0.f) * w4[mb][r];
+        for (int r = 0; r < 16; ++r) s += relu(c3[mb][r]) * w4[mb][r];
     return s + __shfl_xor(s, 32) + b4;
 }
 
@@ -163,9 +173,9 @@ __device__ __forceinline__ f32x16 unit_image(const float *v, int mb, int lane) {
     return r;
 }
 
-// (at least 3 waves per SIMD: keeps the compiler from parking accumulators in
-// AGPRs beyond the 168 registers that allows; the kernel needs 126)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+// (4 waves per SIMD, i.e. at most 128 registers: left free, the compiler
+// parks accumulators in AGPRs and runs 1 wave per SIMD; the kernel needs 126)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_forward(const uint8_t *__restrict__ packed, const uint16_t *__restrict__ x,
                                                  float *__restrict__ y, uint32_t n, float scale, float floor,
                                                  int closure) {
@@ -235,10 +245,10 @@ k_forward(const uint8_t *__restrict__ packed, const uint16_t *__restrict__ x,
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float4 w = *reinterpret_cast<const float4 *>(u0 + 3 * H + mb * 32 + (lane / 32) * 4 + j * 8);
-                o += __builtin_fmaxf(c[mb][4 * j + 0], 0.f) * w.x;
-                o += __builtin_fmaxf(c[mb][4 * j + 1], 0.f) * w.y;
-                o += __builtin_fmaxf(c[mb][4 * j + 2], 0.f) * w.z;
-                o += __builtin_fmaxf(c[mb][4 * j + 3], 0.f) * w.w;
+                o += relu(c[mb][4 * j + 0]) * w.x;
+                o += relu(c[mb][4 * j + 1]) * w.y;
+                o += relu(c[mb][4 * j + 2]) * w.z;
+                o += relu(c[mb][4 * j + 3]) * w.w;
             }
         o = o + __shfl_xor(o, 32) + b4;  // the association of out_dot
         if (lane < 32 && ok) y[cell] = closure ? __builtin_fmaxf(scale * o, floor) : o;

@@ -219,8 +219,9 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 //                the strain rate S = -3 Pi / (2 rho tau) (scaled), phi and
 //                the scaled |grad phi-hat|, as 8 bf16 in one 16-byte store.
 __device__ __forceinline__ uint32_t bf16_pair(float a, float b) {
-    const __bf16 lo = static_cast<__bf16>(a), hi = static_cast<__bf16>(b);
-    return uint32_t(__builtin_bit_cast(uint16_t, lo)) | (uint32_t(__builtin_bit_cast(uint16_t, hi)) << 16);
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));  // one v_cvt_pk_bf16_f32
 }
 template <class MP>
 __device__ __forceinline__ float closure_tau(const KParams &k, MP macp, const Cell &m, uint32_t c, float ph,
