@@ -533,9 +533,11 @@ def calibrate_main(args) -> int:
     LEARNED run of a training jet against that jet's filtered fine run; the
     result is scored on the held-out jet of an evaluation report.
 
-    With --capillary a third parameter s scales the coarse run's surface
-    tension (sigma_coarse = s sigma): a coarse-grid capillary closure, the
-    lever the interface statistics answer to and tau_t cannot reach."""
+    With --capillary a parameter s scales the coarse run's surface tension
+    (sigma_coarse = s sigma), with --sharpening a parameter h its D3Q7
+    sharpening coefficient (gamma_coarse = h gamma): coarse-grid closures of
+    the capillary force and of the phase-field flux, the levers the interface
+    statistics answer to and tau_t cannot reach."""
     from scipy.optimize import minimize
 
     out = Path(args.out)
@@ -550,49 +552,58 @@ def calibrate_main(args) -> int:
     fine_train = run_fine(g, train, steps, every, stat_from, [], z_range, kmax)
     history = []
 
-    def errors(jet, fine, kappa, beta, s=1.0):
+    def errors(jet, fine, kappa, beta, s=1.0, h=1.0):
         net = Network(interface_network(kappa, beta),
                       scales=Scales(strain_scale=1.0, grad_scale=args.grad_scale, tau_scale=1e-3, tau_floor=floor))
 
         def edit(p):
             p.sigma = p.sigma * s
+            p.gamma = p.gamma * h
 
         r = run_coarse(g, jet, abi.CLOSURE_LEARNED, net, steps, every, stat_from, z_range, kmax, tune=False,
-                       edit=edit if s != 1.0 else None)
+                       edit=edit if (s, h) != (1.0, 1.0) else None)
         e = compare(r, fine, z_range, kmax) if r["finite"] else None
         return e, r
 
+    # parameter vector: kappa, beta, then s (--capillary), then h (--sharpening)
+    names = ["kappa", "beta"] + (["s"] if args.capillary else []) + (["h"] if args.sharpening else [])
+
+    def unpack(v):
+        d = dict(zip(names, (float(a) for a in v)))
+        return d["kappa"], d["beta"], d.get("s", 1.0), d.get("h", 1.0)
+
     def objective(v):
-        s = float(v[2]) if args.capillary else 1.0
-        if not 0.5 <= s <= 2.0:  # outside the range the sensitivity study covers
+        kappa, beta, s, h = unpack(v)
+        if not (0.5 <= s <= 2.0 and 0.25 <= h <= 2.0):  # keep to stable, studied ranges
             return 10.0
-        e, r = errors(train, fine_train, float(v[0]), float(v[1]), s)
+        e, r = errors(train, fine_train, kappa, beta, s, h)
         j = max(e.values()) if e else 10.0
-        history.append({"kappa": float(v[0]), "beta": float(v[1]), "sigma_scale": s, "errors": e, "objective": j,
-                        "tau_t_mean": r.get("tau_t_mean"), "tau_t_max": r.get("tau_t_max")})
-        log(f"kappa {v[0]:8.3f} beta {v[1]:8.3f} s {s:6.3f}: max error {j:.4f} {e}")
+        history.append({"kappa": kappa, "beta": beta, "sigma_scale": s, "gamma_scale": h, "errors": e,
+                        "objective": j, "tau_t_mean": r.get("tau_t_mean"), "tau_t_max": r.get("tau_t_max")})
+        log(f"kappa {kappa:8.3f} beta {beta:8.3f} s {s:6.3f} h {h:6.3f}: max error {j:.4f} {e}")
         return j
 
-    x0 = [args.kappa0, args.beta0] + ([1.0] if args.capillary else [])
+    x0 = [args.kappa0, args.beta0] + ([1.0] if args.capillary else []) + ([1.0] if args.sharpening else [])
+    steps0 = {"kappa": args.step, "beta": args.step, "s": args.sigma_step, "h": args.gamma_step}
     simplex = [list(x0)]
-    for i in range(len(x0)):
+    for i, nm in enumerate(names):
         v = list(x0)
-        v[i] += args.sigma_step if i == 2 else args.step
+        v[i] += steps0[nm]
         simplex.append(v)
     res = minimize(objective, x0=np.array(x0), method="Nelder-Mead",
                    options={"maxfev": args.evals, "initial_simplex": np.array(simplex)})
-    kappa, beta = float(res.x[0]), float(res.x[1])
-    s_best = float(res.x[2]) if args.capillary else 1.0
+    kappa, beta, s_best, h_best = unpack(res.x)
     held_fine = ref["filtered_fine"]
-    e_best, _ = errors(held, held_fine, kappa, beta, s_best)
+    e_best, _ = errors(held, held_fine, kappa, beta, s_best, h_best)
     e_none = ref["errors"]["none"]
     rep = {"train_jet": asdict(train), "held_out_jet": asdict(held), "floor": floor, "grad_scale": args.grad_scale,
            "history": history,
-           "best": {"kappa": kappa, "beta": beta, "sigma_scale": s_best, "train_objective": float(res.fun)},
+           "best": {"kappa": kappa, "beta": beta, "sigma_scale": s_best, "gamma_scale": h_best,
+                    "train_objective": float(res.fun)},
            "held_out": {"calibrated": e_best, "none": e_none}}
     (out / "calibrate_report.json").write_text(json.dumps(rep, indent=1))
-    log(f"best kappa {kappa:.3f} beta {beta:.3f} sigma scale {s_best:.3f}; held-out calibrated {e_best}; "
-        f"none {e_none}")
+    log(f"best kappa {kappa:.3f} beta {beta:.3f} sigma scale {s_best:.3f} gamma scale {h_best:.3f}; "
+        f"held-out calibrated {e_best}; none {e_none}")
     return 0
 
 
@@ -647,6 +658,9 @@ def main(argv=None) -> int:
     ca.add_argument("--capillary", action="store_true",
                     help="also fit s, the coarse surface-tension scale (sigma_coarse = s sigma)")
     ca.add_argument("--sigma-step", type=float, default=0.25, help="initial simplex step of s")
+    ca.add_argument("--sharpening", action="store_true",
+                    help="also fit h, the coarse D3Q7 sharpening scale (gamma_coarse = h gamma)")
+    ca.add_argument("--gamma-step", type=float, default=-0.25, help="initial simplex step of h")
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("the closure study runs on the GPU", file=sys.stderr)
