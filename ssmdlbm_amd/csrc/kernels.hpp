@@ -323,7 +323,8 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
     }
     auto fetch_plane = [&](int p, float (&r)[EPT]) {
         const bool plane = p >= 1 && p <= k.nz - 2;
-        const char *base = reinterpret_cast<const char *>(phi + size_t(p) * k.sxy);
+        // (a plane outside the grid reads nothing: its base is never formed)
+        const char *base = reinterpret_cast<const char *>(phi + size_t(plane ? p : 0) * k.sxy);
 #pragma unroll
         for (int q = 0; q < EPT; ++q)
             r[q] = plane && hoff[q] != ~0u ? *reinterpret_cast<const float *>(base + hoff[q]) : 0.f;
