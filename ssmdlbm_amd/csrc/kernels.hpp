@@ -43,6 +43,9 @@ struct KParams {
     // (water inlet), bit 1 y = ny-2 (outflow y), bit 2 z = 1 (oil inlet),
     // bit 3 z = nz-2 (outflow z)
     int bc_planes;
+    // the lattice kernel stores g through the caches (k_phase reads it next)
+    // when one step's g fits the 256 MB Infinity Cache, else non-temporally
+    int g_cached;
     int closure;                 // SSMD_CLOSURE_*
     float smag;                  // 18 Cs^2
     float strain_scale, grad_scale;
@@ -563,8 +566,11 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         phase_moments(ph, sharp, m.ux, m.uy, m.uz, nx, ny, nz, [&](int kk, float v) {
             if (kk > 0) gplane += GB;
             asm volatile("" : "+s"(gplane));
-            __builtin_nontemporal_store(active ? v : 0.f,
-                                        reinterpret_cast<__attribute__((address_space(1))) float *>(gplane + go));
+            auto *dst = reinterpret_cast<__attribute__((address_space(1))) float *>(gplane + go);
+            if (k.g_cached)
+                *dst = active ? v : 0.f;
+            else
+                __builtin_nontemporal_store(active ? v : 0.f, dst);
         });
 
         // rho, u, Pi on the planes the inlet / outflow kernels read them from
