@@ -22,10 +22,11 @@ def small_ensemble(n=5, q=19, pop=abi.POP_F32):
     return out
 
 
-@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("q,pop", [(19, abi.POP_F32), (27, abi.POP_F32), (19, abi.POP_F16), (27, abi.POP_BF16)])
 @pytest.mark.parametrize("graph", [False, True])
-def test_ensemble_members_bitwise_equal_solo(gpu, q, graph):
-    members = small_ensemble(q=q)
+def test_ensemble_members_bitwise_equal_solo(gpu, q, pop, graph):
+    # 16-bit D3Q27 runs the persistent kernel with two planes of pulls in flight
+    members = small_ensemble(q=q, pop=pop)
     ens = Ensemble([c.params() for c in members])
     ens.init()
     steps = 30
