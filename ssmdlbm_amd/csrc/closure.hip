@@ -175,7 +175,7 @@ __device__ __forceinline__ f32x16 unit_image(const float *v, int mb, int lane) {
 
 // (4 waves per SIMD, i.e. at most 128 registers: left free, the compiler
 // parks accumulators in AGPRs and runs 1 wave per SIMD; the kernel needs 126)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_forward(const uint8_t *__restrict__ packed, const uint16_t *__restrict__ x,
                                                  float *__restrict__ y, uint32_t n, float scale, float floor,
                                                  int closure) {
@@ -208,50 +208,61 @@ k_forward(const uint8_t *__restrict__ packed, const uint16_t *__restrict__ x,
 
     const uint32_t tiles = (n + 31) / 32;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = gridDim.x * blockDim.x / 64;
-    for (uint32_t t = wave; t < tiles; t += nwaves) {
-        const uint32_t cell = t * 32 + uint32_t(lane & 31);
-        const bool ok = cell < n;
-        const bf16x8 xin = load_x(x, cell, lane < 32 && ok);
+    // Two tiles per iteration, A and B, in an order that puts one tile's VALU
+    // epilogue (bf16 packing, ReLU, the output dot product) next to the other
+    // tile's matrix products, which the two pipes can then overlap.  Each
+    // tile's arithmetic is that of a single tile.
+    for (uint32_t t = wave; t < tiles; t += 2 * nwaves) {
+        const uint32_t cellA = t * 32 + uint32_t(lane & 31), cellB = cellA + nwaves * 32;
+        const bool okA = cellA < n, okB = cellB < n;
+        const bf16x8 xA = load_x(x, cellA, lane < 32 && okA), xB = load_x(x, cellB, lane < 32 && okB);
         // an offset the compiler cannot see is 0: the LDS reads below stay in
         // the loop instead of being hoisted into 128 registers
         int z0 = 0;
         asm volatile("" : "+v"(z0));
         const float *u0 = &units[0][0] + z0;
-        f32x16 c[2];
-        bf16x8 b[4];
+        f32x16 cA[2], cB[2];
+        bf16x8 bA[4], bB[4];
+        auto hidden = [&](const bf16x8 (&a)[2][4], const float *bias, const bf16x8 (&b)[4], f32x16 (&c)[2]) {
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb) c[mb] = mfma(a1[mb], xin, unit_image(u0, mb, lane));
-        chunks<true>(c, b);
-        // each block's bias image is read just before its chain of products
+            for (int mb = 0; mb < 2; ++mb) {
+                f32x16 acc = unit_image(bias, mb, lane);
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            f32x16 acc = unit_image(u0 + H, mb, lane);
-#pragma unroll
-            for (int kc = 0; kc < 4; ++kc) acc = mfma(a2[mb][kc], b[kc], acc);
-            c[mb] = acc;
-        }
-        chunks<true>(c, b);
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            f32x16 acc = unit_image(u0 + 2 * H, mb, lane);
-#pragma unroll
-            for (int kc = 0; kc < 4; ++kc) acc = mfma(a3[mb][kc], b[kc], acc);
-            c[mb] = acc;
-        }
-        // output: relu(c) . W4, four units of W4 at a time from LDS
-        float o = 0.f;
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float4 w = *reinterpret_cast<const float4 *>(u0 + 3 * H + mb * 32 + (lane / 32) * 4 + j * 8);
-                o += relu(c[mb][4 * j + 0]) * w.x;
-                o += relu(c[mb][4 * j + 1]) * w.y;
-                o += relu(c[mb][4 * j + 2]) * w.z;
-                o += relu(c[mb][4 * j + 3]) * w.w;
+                for (int kc = 0; kc < 4; ++kc) acc = mfma(a[mb][kc], b[kc], acc);
+                c[mb] = acc;
             }
-        o = o + __shfl_xor(o, 32) + b4;  // the association of out_dot
-        if (lane < 32 && ok) y[cell] = closure ? __builtin_fmaxf(scale * o, floor) : o;
+        };
+        // output: relu(c) . W4, four units of W4 at a time from LDS
+        auto output = [&](const f32x16 (&c)[2]) {
+            float o = 0.f;
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float4 w = *reinterpret_cast<const float4 *>(u0 + 3 * H + mb * 32 + (lane / 32) * 4 + j * 8);
+                    o += relu(c[mb][4 * j + 0]) * w.x;
+                    o += relu(c[mb][4 * j + 1]) * w.y;
+                    o += relu(c[mb][4 * j + 2]) * w.z;
+                    o += relu(c[mb][4 * j + 3]) * w.w;
+                }
+            return o + __shfl_xor(o, 32) + b4;  // the association of out_dot
+        };
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) cA[mb] = mfma(a1[mb], xA, unit_image(u0, mb, lane));
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) cB[mb] = mfma(a1[mb], xB, unit_image(u0, mb, lane));
+        chunks<true>(cA, bA);
+        hidden(a2, u0 + H, bA, cA);
+        chunks<true>(cB, bB);
+        hidden(a2, u0 + H, bB, cB);
+        chunks<true>(cA, bA);
+        hidden(a3, u0 + 2 * H, bA, cA);
+        chunks<true>(cB, bB);
+        hidden(a3, u0 + 2 * H, bB, cB);
+        const float oA = output(cA);
+        const float oB = output(cB);
+        if (lane < 32 && okA) y[cellA] = closure ? __builtin_fmaxf(scale * oA, floor) : oA;
+        if (lane < 32 && okB) y[cellB] = closure ? __builtin_fmaxf(scale * oB, floor) : oB;
     }
 }
 
