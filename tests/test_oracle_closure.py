@@ -163,3 +163,20 @@ def test_interface_network_is_affine_in_the_gradient_feature():
         y = mlp_forward(interface_network(kappa, beta).astype(np.float64), x, quant=True)
         k32, b32 = float(np.float32(kappa)), float(np.float32(beta))  # the parameters are float32
         np.testing.assert_allclose(y, k32 * x[:, 7] + b32, rtol=1e-12, atol=1e-12)
+
+
+def test_calibrate_command_line_accepts_the_closure_parameters(tmp_path, capsys):
+    """The calibration's options parse on any machine; without a GPU the
+    study stops right after parsing (it runs only on the device)."""
+    import torch
+
+    from ssmdlbm_amd import closure_study
+    argv = ["calibrate", "--out", str(tmp_path), "--report", str(tmp_path / "eval_report.json"), "--capillary",
+            "--sharpening", "--kappa0", "-8.2", "--beta0", "-2.0", "--sigma-step", "0.2", "--gamma-step", "-0.2",
+            "--evals", "5"]
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: the command would run the study")
+    assert closure_study.main(argv) == 1
+    assert "runs on the GPU" in capsys.readouterr().err
+    with pytest.raises(SystemExit):
+        closure_study.main(["calibrate", "--out", str(tmp_path), "--report", "r.json", "--capillary-scale", "2"])
