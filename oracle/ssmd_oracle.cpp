@@ -44,8 +44,8 @@
 namespace {
 
 // ---------------------------------------------------------------------------
-// Velocity sets (direction order of D3Q19.cuh:82-131, D3Q27.cuh:274-323,
-// D3Q7.cuh:463-512; weights D3Q19.cuh:50-80, D3Q27.cuh:233-272, D3Q7.cuh:440-461).
+// Velocity sets (direction order of D3Q19.cuh:82-131, D3Q27.cuh:92-140,
+// D3Q7.cuh:74-122; weights D3Q19.cuh:50-80, D3Q27.cuh:50-90, D3Q7.cuh:50-72).
 const int C27[27][3] = {
     {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
     {1, 1, 0},   {-1, -1, 0}, {1, 0, 1},   {-1, 0, -1}, {0, 1, 1},   {0, -1, -1}, {1, -1, 0},
@@ -167,12 +167,12 @@ struct Lat {
         as2 = T(3);
         oos = T(1.0 / 6.0);
     }
-    // D3Q19.cuh:133-140 / D3Q27.cuh:330-337
+    // D3Q19.cuh:133-140 / D3Q27.cuh:147-154
     T feq(int i, T rho, T uu, T cu) const {
         if (q == 19) return w[i] * rho * (T(1) - uu + cu + T(0.5) * cu * cu) - w[i];
         return w[i] * rho * (T(1) - uu + cu + T(0.5) * cu * cu + oos * cu * cu * cu - uu * cu) - w[i];
     }
-    // D3Q19.cuh:142-161 / D3Q27.cuh:339-368
+    // D3Q19.cuh:142-161 / D3Q27.cuh:156-185
     T fneq(int i, T pxx, T pyy, T pzz, T pxy, T pxz, T pyz, T ux, T uy, T uz) const {
         const int cx = C27[i][0], cy = C27[i][1], cz = C27[i][2];
         T s = (T(cx * cx) - cs2) * pxx + (T(cy * cy) - cs2) * pyy + (T(cz * cz) - cs2) * pzz +
@@ -196,11 +196,11 @@ struct Lat {
         return w[i] * ((as2 * (cx - ux) + as2 * cu * cx) * fx + (as2 * (cy - uy) + as2 * cu * cy) * fy +
                        (as2 * (cz - uz) + as2 * cu * cz) * fz);
     }
-    // D3Q7.cuh:514-522
+    // D3Q7.cuh:124-132
     T geq(int k, T phi, T ux, T uy, T uz) const {
         return w7[k] * phi * (T(1) + T(4) * (T(C27[k][0]) * ux + T(C27[k][1]) * uy + T(C27[k][2]) * uz));
     }
-    // D3Q7.cuh:530-538
+    // D3Q7.cuh:140-148
     T antidiff(int k, T sharp, T nx, T ny, T nz) const {
         return w7[k] * sharp * (T(C27[k][0]) * nx + T(C27[k][1]) * ny + T(C27[k][2]) * nz);
     }
@@ -265,7 +265,7 @@ struct Oracle final : Base {
         nny.assign(N, T(0));
         nnz.assign(N, T(0));
         nind.assign(N, T(0));
-        // globalFunctions.cuh:334-385: viscosities and omega are formed in
+        // globalFunctions.cuh:232-283: viscosities and omega are formed in
         // double and cast to the arithmetic type.
         visc_water = T(p.visc_water);
         visc_oil = T(p.visc_oil);

@@ -12,7 +12,7 @@ from dataclasses import dataclass
 from . import abi
 
 # D3Q19 / D3Q27 direction tables (reference velocitySet/D3Q19.cuh:82-131,
-# D3Q27.cuh:274-323); D3Q7 is the first seven entries.
+# D3Q27.cuh:92-140); D3Q7 is the first seven entries.
 DIRECTIONS = (
     (0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
     (1, 1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, -1), (0, 1, 1), (0, -1, -1), (1, -1, 0),

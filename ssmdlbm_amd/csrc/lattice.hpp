@@ -2,7 +2,7 @@
 // for the gfx950 kernels.
 //
 // Direction order and weights are those of the reference velocity sets
-// (src/velocitySet/D3Q19.cuh:50-131, D3Q27.cuh:233-323, D3Q7.cuh:440-512):
+// (src/velocitySet/D3Q19.cuh:50-131, D3Q27.cuh:50-140, D3Q7.cuh:50-122):
 // the boundary kernels address populations by number and opposite
 // directions are the pairs (2k-1, 2k), which the collision below exploits.
 #pragma once
@@ -195,7 +195,7 @@ __device__ __forceinline__ Cell moments(const typename P::T (&f)[Q], float fx, f
 //          = E0 + c_u (rho c_u / 2 + 1.5 c.F) + 4.5 omco h2,
 //       E0 = drho - rho uu - 1.5 u.F;
 //     odd  = rho c_u (1 [+ c_u^2/6 - uu]) + 1.5 c.F [+ 4.5 omco h3],
-// the bracketed terms for D3Q27 (D3Q27.cuh:358-367 for h3).  The per-cell
+// the bracketed terms for D3Q27 (D3Q27.cuh:175-184 for h3).  The per-cell
 // sums are formed once, so a direction pair costs 7-10 operations.
 template <int Q, class Emit>
 __device__ __forceinline__ void collide(const Cell &m, float omco, float fx, float fy, float fz, Emit &&emit) {
@@ -210,7 +210,7 @@ __device__ __forceinline__ void collide(const Cell &m, float omco, float fx, flo
     const float F15[3] = {1.5f * fx, 1.5f * fy, 1.5f * fz};
     const float D[3] = {m.pxx - t, m.pyy - t, m.pzz - t};  // c_a c_a - t, one axis
     const float P2[3] = {2.0f * m.pxy, 2.0f * m.pxz, 2.0f * m.pyz};
-    // third-order coefficient combinations (D3Q27.cuh:358-367)
+    // third-order coefficient combinations (D3Q27.cuh:175-184)
     float axxy = 0.f, axxz = 0.f, axyy = 0.f, ayyz = 0.f, axzz = 0.f, ayzz = 0.f, axyz = 0.f;
     if constexpr (Q == 27) {
         axxy = m.pxx * uy + 2.0f * ux * m.pxy;
@@ -275,7 +275,7 @@ __device__ __forceinline__ void collide(const Cell &m, float omco, float fx, flo
     });
 }
 
-// D3Q7 post-collision phase populations (D3Q7.cuh:514-538, lbm.cu:256-269):
+// D3Q7 post-collision phase populations (D3Q7.cuh:124-148, lbm.cu:256-269):
 //     g_k = w_k phi (1 + 4 c_k.u) + w_k gamma phi (1-phi) c_k.n = w_k (phi + c_k.o),
 //     o = 4 phi u + gamma phi (1-phi) n.
 // All seven are fixed by (phi, o), which is what is stored: emit(0, phi),

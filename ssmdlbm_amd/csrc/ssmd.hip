@@ -41,7 +41,7 @@ KParams make_kparams(const ssmd_params &p) {
     k.nz = int(p.nz);
     k.sxy = p.nx * p.ny;
     k.N = p.nx * p.ny * p.nz;
-    // relaxation:: (globalFunctions.cuh:334-385): formed in double, cast.
+    // relaxation:: (globalFunctions.cuh:232-283): formed in double, cast.
     k.visc_water = float(p.visc_water);
     k.visc_diff = float(p.visc_oil) - float(p.visc_water);
     auto omega = [](double nu) { return 1.0 / (0.5 + 3.0 * nu); };
