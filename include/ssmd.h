@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SSMD_ABI_VERSION 4
+#define SSMD_ABI_VERSION 5
 
 #define SSMD_OK 0
 #define SSMD_EINVAL (-1)       /* bad argument or inconsistent parameters */
@@ -86,7 +86,8 @@ enum { SSMD_CLOSURE_NONE = 0, SSMD_CLOSURE_SMAGORINSKY = 1, SSMD_CLOSURE_LEARNED
  * oil:   face z = 0, disc coordinates (a, b) = (x, y)   (applyOilInflow,   BoundaryConditions.cuh:101-168) */
 typedef struct ssmd_jet {
     int32_t enabled;
-    int32_t pad0;
+    int32_t diameter; /* mesh::diam_* (constants.cuh:77-78): nominal diameter in cells, used
+                         for the info file's Reynolds and Weber numbers; 0 = 2 sqrt(r2) */
     double center_a; /* geometry::center_x() = (nx-1)/2 in the reference */
     double center_b; /* geometry::z_pos() / y_pos() in the reference */
     double r2;       /* geometry::R2_*(): (D/2)^2 with integer D/2 */
@@ -264,7 +265,10 @@ int ssmd_ensemble_destroy(ssmd_ensemble *e);
  *                    missing or wrong-size .bin files are skipped
  *                    (host::PostProcess::vti, PostProcess.cuh:101-262)
  *   ssmd_write_info  <dir>/<id>_info.txt metadata
- *                    (host::generateSimulationInfoFile, functions/hostFunctions.cuh:47-87) */
+ *                    (host::generateSimulationInfoFile, functions/hostFunctions.cuh:47-87);
+ *                    the Reynolds numbers u D / nu and the Weber number
+ *                    u_oil^2 D_oil / sigma are formed from the parameters
+ *                    (D = ssmd_jet.diameter) and print as integers when they are */
 int ssmd_write_bin(const ssmd_params *p, const float *device_field, const char *dir, const char *name, uint64_t step,
                    void *stream);
 int ssmd_write_vti(const ssmd_params *p, const char *dir, uint64_t step, const char *const *names, int32_t count);
@@ -293,11 +297,12 @@ int ssmd_derived_update(const ssmd_params *p, const ssmd_fields *d, const ssmd_d
  * Checkpoint / restart (no counterpart in the reference).  The file holds the
  * state one step hands to the next -- the populations the next step reads
  * (f, g), phi (its ghost layer is boundary data), the moment fields and, for
- * the LEARNED closure, tau_t -- with the step index and the parameters.
- * ssmd_checkpoint_read initializes the fields (ssmd_init: ghost slots no
- * kernel writes keep their t = 0 values) and then loads that state; continuing
- * from `step` reproduces the uninterrupted run bit for bit.  Both calls
- * synchronize `stream`. */
+ * the LEARNED closure, tau_t and the packed network -- with the step index
+ * and the parameters.  ssmd_checkpoint_read returns SSMD_EINVAL unless *p
+ * equals the file's parameters in everything but the launch shape (tile, lz);
+ * it then initializes the fields (ssmd_init: ghost slots no kernel writes keep
+ * their t = 0 values) and loads that state; continuing from `step` reproduces
+ * the uninterrupted run bit for bit.  Both calls synchronize `stream`. */
 int ssmd_checkpoint_write(const ssmd_params *p, const ssmd_fields *d, uint64_t step, const char *path, void *stream);
 int ssmd_checkpoint_read(const ssmd_params *p, ssmd_fields *d, uint64_t *step, const char *path, void *stream);
 

@@ -14,7 +14,7 @@ POP_F32, POP_F16, POP_BF16 = 0, 1, 2
 WRAP_NONE, WRAP_REFERENCE, WRAP_TRUE = 0, 1, 2
 CLOSURE_NONE, CLOSURE_SMAGORINSKY, CLOSURE_LEARNED = 0, 1, 2
 MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 8, 64, 8961
-ABI_VERSION = 4
+ABI_VERSION = 5
 STEP_MACROS = 1
 REDUCE_DOUBLES = 8192
 POP_BYTES = {POP_F32: 4, POP_F16: 2, POP_BF16: 2}
@@ -23,7 +23,7 @@ POP_BYTES = {POP_F32: 4, POP_F16: 2, POP_BF16: 2}
 class Jet(C.Structure):
     _fields_ = [
         ("enabled", C.c_int32),
-        ("pad0", C.c_int32),
+        ("diameter", C.c_int32),
         ("center_a", C.c_double),
         ("center_b", C.c_double),
         ("r2", C.c_double),

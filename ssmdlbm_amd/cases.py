@@ -86,6 +86,7 @@ class JetCase:
         p.oil.center_b = (self.ny - 1) * 0.5
         r = self.diameter // 2  # integer radius as mesh::radius_* (constants.cuh:80-81)
         p.oil.r2 = float(r * r)
+        p.oil.diameter = self.diameter
         p.oil.u = self.u
         p.oil.phi = 1.0
         p.outflow_y = 0

@@ -95,6 +95,7 @@ def coarse_params(g: Geometry, jet: Jet, closure: int = abi.CLOSURE_NONE,
     p.oil.center_a, p.oil.center_b = (g.nx - 1) * 0.5, (g.ny - 1) * 0.5
     r = g.diameter // 2
     p.oil.r2 = float(r * r)
+    p.oil.diameter = g.diameter
     p.outflow_z = 1
     p.wrap_x = p.wrap_y = abi.WRAP_REFERENCE
     p.closure, p.cs = closure, g.cs
@@ -122,6 +123,7 @@ def fine_params(g: Geometry, jet: Jet) -> abi.Params:
     p.oil.center_a, p.oil.center_b = 2.0 * c.oil.center_a - 0.5, 2.0 * c.oil.center_b - 0.5
     r = g.diameter  # 2 * (D // 2) with D even
     p.oil.r2 = float(r * r)
+    p.oil.diameter = 2 * g.diameter
     p.seed = jet.seed + 7919
     p.closure = abi.CLOSURE_NONE
     return p

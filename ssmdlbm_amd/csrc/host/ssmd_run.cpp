@@ -61,6 +61,7 @@ void jet(ssmd_params &p, uint32_t nx, uint32_t ny, uint32_t nz, int q, double si
     p.oil.center_a = (nx - 1) * 0.5;
     p.oil.center_b = (ny - 1) * 0.5;
     p.oil.r2 = double((diameter / 2) * (diameter / 2));
+    p.oil.diameter = diameter;
     p.oil.u = u;
     p.oil.phi = 1.0;
     p.outflow_z = 1;

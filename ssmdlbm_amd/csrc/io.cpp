@@ -10,6 +10,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -30,8 +31,9 @@ std::string step6(uint64_t step) {
     return ss.str();
 }
 
-// Restart file: header, then the buffers below in order, raw.
-constexpr char kMagic[8] = {'S', 'S', 'M', 'D', 'C', 'K', 'P', '1'};
+// Restart file: header, then the buffers below in order, raw (for the
+// LEARNED closure the last one is the packed network).
+constexpr char kMagic[8] = {'S', 'S', 'M', 'D', 'C', 'K', 'P', '2'};
 struct CkptHeader {
     char magic[8];
     int32_t abi, params_bytes;
@@ -44,21 +46,48 @@ struct Buf {
     size_t bytes;
 };
 
-// f, g as the next step reads them, phi, the 13 moment fields, tau_t.
+// f, g as the next step reads them, phi, the 13 moment fields; for the
+// LEARNED closure also tau_t and the packed network the next step applies.
 std::vector<Buf> state_buffers(const ssmd_params &p, const ssmd_fields &d) {
     const size_t n = size_t(p.nx) * p.ny * p.nz;
     const size_t sf = p.pop_format == SSMD_POP_F32 ? 4 : 2;
     std::vector<Buf> b = {{d.f, n * size_t(p.q) * sf}, {d.g, n * 4 * 4}};
     float *scal[] = {d.phi, d.rho, d.ux, d.uy, d.uz, d.pxx, d.pyy, d.pzz, d.pxy, d.pxz, d.pyz, d.fsx, d.fsy, d.fsz};
     for (float *q : scal) b.push_back({q, n * 4});
-    if (p.closure == SSMD_CLOSURE_LEARNED) b.push_back({d.tau_t, n * 4});
+    if (p.closure == SSMD_CLOSURE_LEARNED) {
+        b.push_back({d.tau_t, n * 4});
+        b.push_back({d.mlp, ssmd_mlp_packed_bytes()});
+    }
     return b;
 }
 
-// The parameters that fix the state's shape and meaning (not the launch shape).
+bool same_jet(const ssmd_jet &a, const ssmd_jet &b) {
+    return a.enabled == b.enabled && a.diameter == b.diameter && a.center_a == b.center_a &&
+           a.center_b == b.center_b && a.r2 == b.r2 && a.u == b.u && a.phi == b.phi;
+}
+
+// Every parameter of the case; only the launch shape (tile, lz) may differ.
 bool same_case(const ssmd_params &a, const ssmd_params &b) {
     return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.q == b.q && a.pop_format == b.pop_format &&
-           a.closure == b.closure;
+           a.outflow_y == b.outflow_y && a.outflow_z == b.outflow_z && a.wrap_x == b.wrap_x &&
+           a.wrap_y == b.wrap_y && a.wrap_z == b.wrap_z && a.visc_water == b.visc_water &&
+           a.visc_oil == b.visc_oil && a.sigma == b.sigma && a.gamma == b.gamma && a.perturb == b.perturb &&
+           a.seed == b.seed && same_jet(a.water, b.water) && same_jet(a.oil, b.oil) && a.closure == b.closure &&
+           a.cs == b.cs && a.strain_scale == b.strain_scale && a.grad_scale == b.grad_scale &&
+           a.tau_scale == b.tau_scale && a.tau_floor == b.tau_floor;
+}
+
+// Nominal jet diameter: the stated one, else the even diameter of the disc.
+double jet_diameter(const ssmd_jet &j) { return j.diameter > 0 ? double(j.diameter) : 2.0 * std::floor(std::sqrt(j.r2) + 1e-9); }
+
+// A dimensionless number as the reference prints its integer constants
+// (constants.cuh:86-90): an integer when it is one to 1e-6, else the value.
+std::string dimensionless(double v) {
+    std::ostringstream ss;
+    const double r = std::round(v);
+    if (std::isfinite(v) && std::fabs(v - r) <= 1e-6 * std::fabs(v)) ss << static_cast<long long>(r);
+    else ss << v;
+    return ss.str();
 }
 
 constexpr size_t kChunk = size_t(64) << 20;
@@ -210,18 +239,28 @@ int ssmd_write_info(const ssmd_params *p, const char *dir, const char *id, const
     const std::filesystem::path path = std::filesystem::path(dir) / (std::string(id) + "_info.txt");
     std::ofstream f(path, std::ios::out | std::ios::trunc);
     if (!f) return SSMD_EINVAL;
-    auto diam = [](const ssmd_jet &j) { return 2 * int(std::sqrt(j.r2) + 1e-9); };
+    // The reference prints its compile-time Reynolds and Weber constants; here
+    // they follow from the runtime parameters by inverting the formulas that
+    // set the viscosities and sigma: nu = u D / Re (globalFunctions.cuh:234-242)
+    // and sigma = u_oil^2 D_oil / We (constants.cuh:91).  A disabled jet prints 0.
+    auto reynolds = [](const ssmd_jet &j, double nu) {
+        return j.enabled ? dimensionless(j.u * jet_diameter(j) / nu) : std::string("0");
+    };
+    const std::string weber =
+        p->oil.enabled && p->sigma > 0.0 ? dimensionless(p->oil.u * p->oil.u * jet_diameter(p->oil) / p->sigma)
+                                         : std::string("0");
+    auto diam = [](const ssmd_jet &j) { return j.enabled ? static_cast<long long>(jet_diameter(j)) : 0LL; };
     f << "---------------------------- SIMULATION METADATA ----------------------------\n"
       << "ID:                 " << id << '\n'
       << "Velocity set:       " << velocity_set << '\n'
       << "Water velocity:     " << (p->water.enabled ? float(p->water.u) : 0.0f) << '\n'
       << "Oil velocity:       " << (p->oil.enabled ? float(p->oil.u) : 0.0f) << '\n'
-      << "Water viscosity:    " << p->visc_water << '\n'
-      << "Oil viscosity:      " << p->visc_oil << '\n'
-      << "Surface tension:    " << float(p->sigma) << "\n\n"
+      << "Water Reynolds:     " << reynolds(p->water, p->visc_water) << '\n'
+      << "Oil Reynolds:       " << reynolds(p->oil, p->visc_oil) << '\n'
+      << "Weber number:       " << weber << "\n\n"
       << "Domain size:        NX=" << p->nx << ", NY=" << p->ny << ", NZ=" << p->nz << '\n'
-      << "Water diameter:     D=" << (p->water.enabled ? diam(p->water) : 0) << '\n'
-      << "Oil diameter:       D=" << (p->oil.enabled ? diam(p->oil) : 0) << '\n'
+      << "Water diameter:     D=" << diam(p->water) << '\n'
+      << "Oil diameter:       D=" << diam(p->oil) << '\n'
       << "Timesteps:          " << nsteps << '\n'
       << "Output interval:    " << save_every << "\n\n"
       << "-----------------------------------------------------------------------------\n";

@@ -299,12 +299,14 @@ int ssmd_params_reference_default(ssmd_params *p, int32_t q) {
     p->water.center_a = cx;
     p->water.center_b = double(0.7f * float(cz));
     p->water.r2 = double(radius * radius);
+    p->water.diameter = diam;
     p->water.u = u_water;
     p->water.phi = 0.0;
     p->oil.enabled = 1;
     p->oil.center_a = cx;
     p->oil.center_b = double(0.5f * float(cy));
     p->oil.r2 = double(radius * radius);
+    p->oil.diameter = diam;
     p->oil.u = u_oil;
     p->oil.phi = 1.0;
     p->outflow_y = 1;

@@ -150,3 +150,33 @@ def test_error_paths_without_device():
     assert lib.ssmd_error_string(-3) == b"fields or graph not ready"
     with pytest.raises(abi.SsmdError):
         abi.check(-2, "x")
+
+
+def test_info_file_reference_lines(tmp_path):
+    """The info file of the reference's own case carries its metadata lines
+    (hostFunctions.cuh:63-76): Reynolds 1400 / 450 and Weber 500 come back
+    from nu = u D / Re and sigma = u_oil^2 D_oil / We, diameters 13."""
+    lib = abi.load()
+    p = cases.reference_default(19)
+    assert (p.water.diameter, p.oil.diameter) == (13, 13)
+    assert lib.ssmd_write_info(p, str(tmp_path).encode(), b"007", b"D3Q19", 100000, 1000) == 0
+    lines = (tmp_path / "007_info.txt").read_text().splitlines()
+    want = {
+        "ID:": "007", "Velocity set:": "D3Q19", "Water velocity:": "0.05", "Oil velocity:": "0.05",
+        "Water Reynolds:": "1400", "Oil Reynolds:": "450", "Weber number:": "500",
+        "Domain size:": "NX=128, NY=256, NZ=256", "Water diameter:": "D=13", "Oil diameter:": "D=13",
+        "Timesteps:": "100000", "Output interval:": "1000",
+    }
+    got = {}
+    for line in lines:
+        for key in want:
+            if line.startswith(key):
+                got[key] = line[len(key):].strip()
+    assert got == want
+    assert lines[0].startswith("---") and lines[-1].startswith("---")
+    # a single oil jet: no water jet (Reynolds 0), Re = u D / nu = 0.05 * 8 / (1/60) = 24
+    j = cases.JET_D3Q19_SMALL.params()
+    assert lib.ssmd_write_info(j, str(tmp_path).encode(), b"jet", b"D3Q19", 10, 5) == 0
+    text = (tmp_path / "jet_info.txt").read_text()
+    assert "Water Reynolds:     0\n" in text and "Oil Reynolds:       24\n" in text
+    assert "Weber number:       0.2\n" in text and "Oil diameter:       D=8\n" in text  # 0.05^2 * 8 / 0.1
