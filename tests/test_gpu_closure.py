@@ -174,12 +174,17 @@ def test_smagorinsky_jet_matches_oracle(gpu, q):
     assert rel_l2(plain.field("uz"), sim.field("uz")) > 1e-4
 
 
-def learned_case():
-    return closure_jet(abi.CLOSURE_LEARNED, strain_scale=200.0, grad_scale=4.0, tau_scale=0.02)
+def learned_case(q=19):
+    p = closure_jet(abi.CLOSURE_LEARNED, strain_scale=200.0, grad_scale=4.0, tau_scale=0.02)
+    p.q = q
+    return p
 
 
-def test_learned_closure_inputs_and_update(gpu):
-    p = learned_case()
+@pytest.mark.parametrize("q", [19, 27])
+def test_learned_closure_inputs_and_update(gpu, q):
+    """The LEARNED step on D3Q19 and on D3Q27 (the coarse lattice of the
+    learned-closure workload, whose third-order collision reads the same tau)."""
+    p = learned_case(q)
     params = random_params(21, bias=0.2)
     net = Network(params, scales=Scales(200.0, 4.0, 0.02))
     sim = Simulation(p)
@@ -359,6 +364,47 @@ def test_learned_closure_beats_no_closure_and_smagorinsky_a_priori(gpu, scaling)
           f"{x.shape[0]} training samples, target mean {float((w * t).mean()):.3f}")
     assert learned < 0.9 * none
     assert learned < 0.9 * smag_err
+
+
+@pytest.mark.slow
+def test_learned_closure_at_the_coarse_box(gpu):
+    """The learned-closure workload's coarse run: D3Q27 + D3Q7 on 128x128x512
+    at its stated tau = 0.51 + tau_t, with the shipped network and its scales.
+    Over 20 steps the fields stay finite, tau_t stays at or above its floor
+    (0), and the graph replay
+    equals the eager steps bit for bit (fields, tau_t and both population sets)."""
+    from pathlib import Path
+    import ssmdlbm_amd
+    from ssmdlbm_amd.closure_study import Geometry, Jet, coarse_params
+    net = Network.load(Path(ssmdlbm_amd.__file__).parent / "data" / "closure_jet_d3q27")
+    g = Geometry(tau=0.51)
+    p = coarse_params(g, Jet(0.05, 0.05, 5), abi.CLOSURE_LEARNED, net.scales)
+    assert (p.nx, p.ny, p.nz, p.q) == (128, 128, 512, 27)
+    assert p.visc_water == pytest.approx((0.51 - 0.5) / 3)
+    runs = []
+    for graph in (False, True):
+        sim = Simulation(p)
+        net.load_into(sim)
+        sim.init()
+        print(f"128x128x512 LEARNED, graph={graph}", flush=True)
+        if graph:
+            sim.graph_run(19)
+        else:
+            sim.step(19)
+        sim.step(1, macros=True)
+        sim.synchronize()
+        runs.append({k: sim.field(k) for k in ("rho", "ux", "uy", "uz", "phi", "tau_t")})
+        runs[-1]["f"] = sim.pops_source("f")
+        runs[-1]["g"] = sim.g_moments()
+        sim.close()
+        del sim
+    a, b = runs
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("rho", "ux", "uy", "uz", "phi", "tau_t"):
+        assert np.isfinite(a[k]).all(), k
+    assert a["tau_t"].min() >= 0.0
+    print(f"coarse LEARNED: tau_t mean {a['tau_t'].mean():.3e}, max {a['tau_t'].max():.3e}")
 
 
 def test_shipped_network_runs_a_learned_jet(gpu):

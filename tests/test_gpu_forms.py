@@ -47,6 +47,57 @@ def test_ensemble_members_bitwise_equal_solo(gpu, q, pop, graph):
         np.testing.assert_array_equal(m.pops_source("g"), solo.pops_source("g"))
 
 
+@pytest.mark.slow
+def test_seeded_32_jet_ensemble_at_its_own_boxes(gpu):
+    """The batching form's workload itself: the 32 seeded D3Q19 jets (boxes
+    48x48x96 .. 128x128x384, 68.6 M cells; u, sigma, diameter per jet), tile
+    chosen by the autotuner, steps replayed from the ensemble's graphs.
+    After 20 steps the smallest jet is within 1e-5 relative L2 of the FP64
+    oracle; after 50 the smallest, the largest and the most elongated jet
+    are bit-identical to their solo runs (fields and both population sets)."""
+    from conftest import cpu_threads
+    from oracle.oracle import Oracle
+
+    jets = cases.ensemble(32, seed=4)
+    ens = Ensemble([c.params() for c in jets])
+    ens.init()
+    tile = ens.tune(reps=2)
+    print(f"32-jet ensemble, {ens.cells / 1e6:.1f} M cells, tile {tile}", flush=True)
+    ens.graph_run(19)
+    ens.step(1, macros=True)
+    ens.synchronize()
+    small = min(range(len(jets)), key=lambda i: jets[i].cells)
+    large = max(range(len(jets)), key=lambda i: jets[i].cells)
+    elongated = max(range(len(jets)), key=lambda i: jets[i].nz / jets[i].nx)
+    assert len({small, large, elongated}) == 3
+    o = Oracle(jets[small].params(), fp64=True, threads=cpu_threads())
+    o.init()
+    o.step(20)
+    m = ens.members[small]
+    for name in MACRO:
+        b = o.field(name)
+        e = np.linalg.norm(m.field(name) - b) / np.linalg.norm(b)
+        assert e < 1e-5, (jets[small].name, name, e)
+    ens.graph_run(29)
+    ens.step(1, macros=True)
+    ens.synchronize()
+    for j in (small, large, elongated):
+        c = jets[j]
+        solo = Simulation(c.params())
+        solo.init()
+        solo.step(19)
+        solo.step(1, macros=True)
+        solo.step(29)
+        solo.step(1, macros=True)
+        m = ens.members[j]
+        for name in MACRO:
+            np.testing.assert_array_equal(m.field(name), solo.field(name), err_msg=f"{c.name} {name}")
+        np.testing.assert_array_equal(m.pops_source("f"), solo.pops_source("f"), err_msg=c.name)
+        np.testing.assert_array_equal(m.g_moments(), solo.g_moments(), err_msg=c.name)
+        solo.close()
+    ens.close()
+
+
 @pytest.mark.parametrize("q,pop", [(19, abi.POP_F32), (27, abi.POP_F32), (27, abi.POP_F16)])
 def test_tile_shapes_agree(gpu, q, pop):
     # two x tiles of 128 (one partial), partial y tiles for every shape; for
@@ -164,7 +215,8 @@ def test_checkpoint_resume_is_bitwise(gpu, tmp_path, closure):
     a.checkpoint(tmp_path / "state.ckpt")
     a.graph_run(20)
     a.step(1, macros=True)
-    b = fresh()
+    b = Simulation(p)  # no network loaded: a LEARNED checkpoint carries it
+    b.init()
     b.step(3)  # the restore must not depend on what the fields held before
     b.restore(tmp_path / "state.ckpt")
     assert b.step_index == 30
@@ -186,6 +238,15 @@ def test_checkpoint_rejects_another_case(gpu, tmp_path):
     b = Simulation(q)
     with pytest.raises(abi.SsmdError):
         b.restore(tmp_path / "s.ckpt")
+    # same box, another physical parameter: rejected; another launch shape: accepted
+    for field, value in (("sigma", p.sigma * 2), ("visc_oil", p.visc_oil * 1.1), ("seed", p.seed + 1)):
+        r = p.copy()
+        setattr(r, field, value)
+        with pytest.raises(abi.SsmdError):
+            Simulation(r).restore(tmp_path / "s.ckpt")
+    r = p.copy()
+    r.tile, r.lz = 3, 8
+    Simulation(r).restore(tmp_path / "s.ckpt")
 
 
 # -- derived statistics ---------------------------------------------------------

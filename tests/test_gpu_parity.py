@@ -309,3 +309,106 @@ def test_reduced_precision_storage(gpu, fmt, tol, q):
     for n in MACRO:
         assert errs[n] < tol, (n, errs[n])
         assert errs[n] < 3 * oerr[n] + 1e-6, (n, errs[n], oerr[n])
+
+
+# -- boundary kernels from a set state --------------------------------------------------
+def _bc_params(q, faces):
+    """Small box with one inlet and one outflow; water and oil viscosities
+    differ, so each face's relaxation constant is visible."""
+    p = abi.Params()
+    p.nx, p.ny, p.nz = 14, 13, 16
+    p.q = q
+    p.visc_water = cases.viscosity_from_tau(0.6)
+    p.visc_oil = cases.viscosity_from_tau(0.9)
+    p.gamma = 0.3
+    if faces == "z":
+        p.oil.enabled, p.oil.r2, p.oil.u, p.oil.phi = 1, 9.0, 0.04, 0.7
+        p.oil.center_a, p.oil.center_b = (p.nx - 1) * 0.5, (p.ny - 1) * 0.5
+        p.outflow_z = 1
+    else:
+        p.water.enabled, p.water.r2, p.water.u, p.water.phi = 1, 4.0, 0.03, 0.7
+        p.water.center_a, p.water.center_b = (p.nx - 1) * 0.5, 6.0
+        p.outflow_y = 1
+    return p
+
+
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("faces", ["z", "y"])
+def test_boundary_kernels_from_a_set_state(gpu, q, faces):
+    """The inlet and outflow kernels alone, driven from one random state
+    (rho, u, phi and traceless Pi on every cell) with distinct water and oil
+    viscosities, against the oracle's boundary kernels from the same state:
+    every population arriving at an interior cell (1e-7 absolute, FP32 vs
+    FP64) and the boundary macros on the ghost faces."""
+    p = _bc_params(q, faces)
+    sim = Simulation(p)
+    sim.init()
+    o = Oracle(p, fp64=True, threads=1)
+    o.init()
+    rng = np.random.default_rng(60 + q + (faces == "y"))
+    shape = p.shape
+    pi = 1e-3 * rng.standard_normal((6,) + shape)
+    pi[:3] -= pi[:3].mean(axis=0)
+    state = {"rho": 1 + 0.01 * rng.standard_normal(shape), "phi": rng.uniform(0, 1, shape)}
+    for k, name in enumerate(("ux", "uy", "uz")):
+        state[name] = 0.02 * rng.standard_normal(shape)
+    for k, name in enumerate(("pxx", "pyy", "pzz", "pxy", "pxz", "pyz")):
+        state[name] = pi[k]
+    for name, v in state.items():
+        v32 = v.astype(np.float32)
+        sim.t[name].copy_(torch.from_numpy(v32.reshape(-1)).to(gpu))
+        o.set_field(name, v32.astype(np.float64))
+    torch.cuda.synchronize()
+    sim.launch_boundaries(macros=True)
+    sim.swap()
+    o.k_boundaries(0)
+    for w in ("f", "g"):
+        a, b = sim.pops_arrival(w), o.pops(w)
+        m = interior_arrivals(a)
+        err = np.nanmax(np.abs(a[m] - b[m]))
+        assert err < 1e-7, (w, err)
+    # the faces' writes are not trivially zero
+    f = o.pops("f")
+    assert np.abs(f[:, 1:-1, 1:-1, 1:-1]).max() > 1e-3
+    for name in MACRO:
+        np.testing.assert_allclose(sim.field(name), o.field(name), rtol=0, atol=1e-7, err_msg=name)
+    sim.close()
+
+
+# -- the throughput boxes themselves ---------------------------------------------------
+@pytest.mark.slow
+def test_d3q27_jet_128x128x512_against_fp64(gpu):
+    """The D3Q27 throughput case at its own size (128x128x512, 8.4 M cells),
+    graph-replayed: every field within 1e-5 relative L2 of the FP64 oracle
+    after 10 steps (the force within 1e-4, as for the small boxes)."""
+    p = cases.JET_D3Q27.params()
+    steps = 10
+    print(f"D3Q27 128x128x512: {steps} GPU steps", flush=True)
+    sim = run_gpu(p, steps, graph=True)
+    print("FP64 oracle", flush=True)
+    o = run_oracle(p, steps)
+    errs = {name: rel_l2(sim.field(name), o.field(name)) for name in ALL}
+    print("128x128x512 D3Q27, 10 steps, relative L2 vs FP64:", errs)
+    for name, e in errs.items():
+        assert e < (1e-4 if name.startswith("fs") else 1e-5), (name, e)
+    sim.close()
+
+
+@pytest.mark.slow
+def test_non_temporal_g_store_box_against_fp64(gpu):
+    """A box above 10.5 M cells (160x160x416): its 16-byte-per-cell g moment
+    planes exceed the 160 MB the lattice kernel stores through the caches, so
+    the step uses the non-temporal g stores of the large boxes.  Fields within
+    1e-5 relative L2 of the FP64 oracle after 10 steps."""
+    p = cases.JET_D3Q19_SMALL.with_(nx=160, ny=160, nz=416, diameter=16, sigma=0.05).params()
+    assert p.cells * 16 > 160 * 2 ** 20
+    steps = 10
+    print(f"D3Q19 160x160x416: {steps} GPU steps", flush=True)
+    sim = run_gpu(p, steps)
+    print("FP64 oracle", flush=True)
+    o = run_oracle(p, steps)
+    errs = {name: rel_l2(sim.field(name), o.field(name)) for name in ALL}
+    print("160x160x416 D3Q19, 10 steps, relative L2 vs FP64:", errs)
+    for name, e in errs.items():
+        assert e < (1e-4 if name.startswith("fs") else 1e-5), (name, e)
+    sim.close()
