@@ -332,6 +332,17 @@ size_t ssmd_mlp_grad_work_bytes(uint32_t n);
  * weighs every sample 1.  Deterministic (fixed reduction order). */
 int ssmd_mlp_grad(const void *packed, const uint16_t *x, const float *t, const float *w, uint32_t n, float *grad,
                   float *loss, void *work, size_t work_bytes, void *stream);
+/* ssmd_mlp_grad without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight training loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * per-block partials go to a library-owned static device array sized for
+ * n <= SSMD_MLP_GRAD_NOALLOC_MAX samples (9.2 MB per device, held until the
+ * process ends); larger n returns SSMD_EINVAL.  Results are bit-identical to
+ * ssmd_mlp_grad.  Every call on one device shares that array: calls that
+ * overlap in time on one device (other streams, other threads) race. */
+#define SSMD_MLP_GRAD_NOALLOC_MAX 65536u
+int ssmd_mlp_grad_noalloc(const void *packed, const uint16_t *x, const float *t, const float *w, uint32_t n,
+                          float *grad, float *loss, void *stream);
 /* One Adam step (Kingma & Ba 2015, bias-corrected; step counts from 1) on
  * params, with first/second moment buffers m, v (SSMD_MLP_PARAMS floats each). */
 int ssmd_mlp_adam(float *params, const float *grad, float *m, float *v, uint32_t step, float lr, float beta1,

@@ -14,6 +14,8 @@ POP_F32, POP_F16, POP_BF16 = 0, 1, 2
 WRAP_NONE, WRAP_REFERENCE, WRAP_TRUE = 0, 1, 2
 CLOSURE_NONE, CLOSURE_SMAGORINSKY, CLOSURE_LEARNED = 0, 1, 2
 MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 8, 64, 8961
+MLP_GRAD_NOALLOC_MAX = 65536
+OK, EINVAL, EUNSUPPORTED, ENOTREADY = 0, -1, -2, -3
 ABI_VERSION = 5
 STEP_MACROS = 1
 REDUCE_DOUBLES = 8192
@@ -150,6 +152,8 @@ SIGNATURES = {
     "ssmd_mlp_grad_work_bytes": (C.c_size_t, [C.c_uint32]),
     "ssmd_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_size_t, C.c_void_p]),
+    "ssmd_mlp_grad_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]),
     "ssmd_mlp_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
                                 C.c_float, C.c_float, C.c_void_p]),
     "ssmd_closure_update": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_void_p]),

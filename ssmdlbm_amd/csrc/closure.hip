@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/ssmd.h"
 #include "lattice.hpp"
@@ -695,11 +696,40 @@ int herr(hipError_t e) { return e == hipSuccess ? 0 : int(e); }
 
 // Blocks of the gradient kernel for n samples: enough 32-sample tiles per
 // block to amortize the per-block partial row, at most one block per tile.
-uint32_t grad_blocks(uint32_t n) {
+constexpr uint32_t grad_blocks(uint32_t n) {
     const uint32_t tiles = (n + 31) / 32;
     uint32_t b = (tiles + 7) / 8;
     if (b > 2048) b = 2048;
     return b < 1 ? 1 : b;
+}
+
+// Workspace of ssmd_mlp_grad_noalloc: the per-block gradient partials for up
+// to SSMD_MLP_GRAD_NOALLOC_MAX samples, one copy per device for the life of
+// the process.  k_grad writes every element of the rows it uses, so nothing
+// needs clearing between calls.
+constexpr uint32_t kNoallocBlocks = grad_blocks(SSMD_MLP_GRAD_NOALLOC_MAX);
+constexpr size_t kNoallocFloats = size_t(kNoallocBlocks) * (SSMD_MLP_PARAMS + 1);
+static __device__ __align__(256) float scratch[kNoallocFloats];
+
+// Device address of `scratch` per device (the host-side name is only a
+// stand-in for it), looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(float **out) {
+    int dev = 0;
+    if (const hipError_t e = hipGetDevice(&dev); e != hipSuccess) return int(e);
+    if (dev < 0 || dev >= kMaxDevices) return SSMD_EUNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)); e != hipSuccess) return int(e);
+        if (!ptr) return SSMD_ENOTREADY;
+        g_scratch_ptr[dev] = static_cast<float *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return SSMD_OK;
 }
 }  // namespace
 
@@ -743,6 +773,22 @@ int ssmd_mlp_grad(const void *packed, const uint16_t *x, const float *t, const f
     if (const int rc = herr(hipGetLastError())) return rc;
     hipLaunchKernelGGL(k_grad_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, s,
                        static_cast<const float *>(work), blocks, n, grad, loss);
+    return herr(hipGetLastError());
+}
+
+int ssmd_mlp_grad_noalloc(const void *packed, const uint16_t *x, const float *t, const float *w, uint32_t n,
+                          float *grad, float *loss, void *stream) {
+    using namespace ssmd::mlp;
+    if (!packed || !x || !t || !grad || !loss || n == 0) return SSMD_EINVAL;
+    if (n > SSMD_MLP_GRAD_NOALLOC_MAX) return SSMD_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return SSMD_EINVAL;
+    float *work = nullptr;
+    if (const int rc = scratch_address(&work); rc != 0) return rc;
+    const uint32_t blocks = grad_blocks(n);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_grad, dim3(blocks), dim3(64), 0, s, static_cast<const uint8_t *>(packed), x, t, w, n, work);
+    if (const int rc = herr(hipGetLastError())) return rc;
+    hipLaunchKernelGGL(k_grad_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, s, work, blocks, n, grad, loss);
     return herr(hipGetLastError());
 }
 
