@@ -256,31 +256,88 @@ def measure_ensemble(steps: int, warmup: int, n_jets: int = 32):
     }
 
 
-def cpu_baseline(case, budget_s: float = 15.0):
+def cpu_info() -> dict:
+    """The host CPU as lscpu reports it: model name, logical CPUs, the CPUs this process may use."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "allowed_cpus": allowed}
+
+
+def _oracle_mlups(params, threads: int, budget_s: float, warmup: int = 2, max_steps: int = 200):
+    """MLUPS of the FP32 oracle over a bounded number of steps (at least 3)."""
     from oracle.oracle import Oracle
 
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        threads = os.cpu_count() or 1
-    threads = max(1, min(16, threads))
-    o = Oracle(case.params(), fp64=False, threads=threads)
+    o = Oracle(params, fp64=False, threads=threads)
     o.init()
     t0 = time.perf_counter()
-    o.step(2)
-    per = (time.perf_counter() - t0) / 2
-    n = int(max(3, min(200, budget_s / max(per, 1e-6))))
+    o.step(warmup)
+    per = (time.perf_counter() - t0) / warmup
+    n = int(max(3, min(max_steps, budget_s / max(per, 1e-6))))
     t0 = time.perf_counter()
     o.step(n)
     wall = time.perf_counter() - t0
-    return {
-        "value": round(case.cells * n / wall / 1e6, 3),
+    return params.cells * n / wall / 1e6, n, wall
+
+
+def cpu_baseline(case, budget_s: float = 15.0, full_small: bool = True):
+    """The oracle's FP32 restatement of the reference schedule (K1-K4 and the
+    boundaries, OpenMP over z) on the host cores: a bounded sample of the
+    bench workload on every allowed core (at most 16, the GPU machines' share)
+    and on one, and the 64x64x128 jet's full 1000 steps."""
+    from ssmdlbm_amd import cases
+
+    # thread placement of the reference CPU baseline plan; read by the oracle's
+    # OpenMP runtime when the oracle library is first loaded (here)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    info = cpu_info()
+    threads = max(1, min(16, info["allowed_cpus"]))
+    mlups, n, wall = _oracle_mlups(case.params(), threads, budget_s)
+    # one thread on the same jet shortened to 64 planes (1 M cells), so that a
+    # few steps take seconds; the CPU rate per cell hardly depends on nz
+    short = case.with_(nz=64)
+    one, n1, _ = _oracle_mlups(short.params(), 1, budget_s / 3, warmup=1, max_steps=20)
+    out = {
+        "value": round(mlups, 3),
         "unit": "MLUPS",
         "cores": threads,
         "kind": "port",
         "sample": f"{case.name}: {n} timed steps after 2 warm-up steps of the oracle's FP32 restatement of the "
                   f"reference schedule (K1-K4 + boundaries), OpenMP over z, {threads} threads",
+        "cpu_model": info["model"],
+        "nproc": info["nproc"],
+        "allowed_cpus": info["allowed_cpus"],
+        "omp_binding": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND", "unset"),
+                        "OMP_PLACES": os.environ.get("OMP_PLACES", "unset")},
+        "single_thread_mlups": round(one, 3),
+        "single_thread_sample": f"{case.name} shortened to {short.nx}x{short.ny}x{short.nz}: {n1} timed steps "
+                                f"after 1 warm-up step, 1 thread",
+        "gbs_at_B_alg": round(mlups * 1e6 * cases.algorithmic_bytes_per_cell(case.q, case.pop_format) / 1e9, 2),
     }
+    if full_small:
+        small = cases.JET_D3Q19_SMALL
+        from oracle.oracle import Oracle
+
+        o = Oracle(small.params(), fp64=False, threads=threads)
+        o.init()
+        o.step(10)  # warm-up, excluded
+        t0 = time.perf_counter()
+        o.step(1000)
+        wall1 = time.perf_counter() - t0
+        out[small.name + "_1000_steps"] = {"seconds": round(wall1, 3),
+                                           "mlups": round(small.cells * 1000 / wall1 / 1e6, 3),
+                                           "threads": threads}
+    return out
 
 
 def copy_bandwidth(reps: int = 10) -> float:
@@ -328,14 +385,40 @@ def profile_mode(case, steps: int, tune: bool) -> int:
     return 0
 
 
+def kernel_source_digest() -> str:
+    """sha256 (16 hex digits) of the lattice kernel sources, which a traffic
+    profile records so that a stale one can be told from a current one."""
+    import hashlib
+
+    h = hashlib.sha256()
+    csrc = ROOT / "ssmdlbm_amd" / "csrc"
+    for name in ("kernels.hpp", "lattice.hpp", "lattice_inst.hpp", "launch.hpp", "ssmd.hip"):
+        h.update((csrc / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
 def load_traffic(workload: str):
+    """(HBM bytes per launch, provenance) from the committed counter profile."""
     path = ROOT / "profiles" / f"k_lattice_{workload}_traffic.json"
-    if path.exists():
-        try:
-            return json.loads(path.read_text()).get("hbm_bytes_per_launch")
-        except (ValueError, OSError):
-            return None
-    return None
+    if not path.exists():
+        return None, None
+    try:
+        d = json.loads(path.read_text())
+    except (ValueError, OSError):
+        return None, None
+    src = d.get("kernel_source_sha256_16")
+    prov = {"profile": str(path.relative_to(ROOT)), "measured_with": d.get("commands"),
+            "kernel_sources_match": src == kernel_source_digest() if src else None}
+    return d.get("hbm_bytes_per_launch"), prov
+
+
+def hbm_probe_mode(sizes_gib) -> int:
+    """The streaming probe alone (for rocprofv3 --kernel-trace): one JSON line per configuration."""
+    from ssmdlbm_amd import probe
+
+    for r in probe.sweep(sizes_gib):
+        print(json.dumps(r), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -349,6 +432,8 @@ def main() -> int:
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16-vs-FP32 storage measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default lattice tile shape")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only the HBM streaming probe (read / write / copy, cached and non-temporal, 1 and 4 GiB)")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling mode for rocprofv3 --pmc: N eager steps of the workload, then a plain "
                          "1 GiB device copy (the byte-count reference), no timing output")
@@ -368,6 +453,8 @@ def main() -> int:
         return 1
     abi.load()
     case = cases.WORKLOADS[args.workload]
+    if args.hbm_probe:
+        return hbm_probe_mode((1, 4))
     if args.profile_steps:
         return profile_mode(case, args.profile_steps, tune=not args.no_tune)
     log(f"workload {case.name}: warmup {args.warmup}, timed {args.steps}")
@@ -375,6 +462,11 @@ def main() -> int:
     log(f"main: {main_m}")
     copy_gbs = copy_bandwidth()
     log(f"device copy: {copy_gbs:.0f} GB/s")
+    from ssmdlbm_amd import probe
+
+    sweep = probe.sweep((1, 4))
+    best = probe.ceiling(sweep)
+    log(f"HBM probe best: { {k: v['gbs'] for k, v in best.items()} }")
 
     forms = None
     if not args.no_forms:
@@ -404,6 +496,8 @@ def main() -> int:
         log(f"cpu: {cpu}")
 
     kb = main_m["kernel_bytes_per_cell"]
+    traffic, traffic_prov = load_traffic(case.name)
+    ceiling_gbs = best["copy"]["gbs"]
     out = {
         "metric": METRIC,
         "value": round(main_m["mlups"], 2),
@@ -425,6 +519,8 @@ def main() -> int:
             "tau": case.tau, "tau_g": case.tau_g, "sigma": case.sigma, "inlet_diameter": case.diameter,
             "u_inlet": case.u, "perturb": case.perturb,
             "timed_region": "HIP graph replay, one graph per step (phase, fused lattice, boundary kernels)",
+            "step_window": f"steps {args.warmup + 3}..{args.warmup + 2 + args.steps} after t = 0 "
+                           f"(2 tuning steps, {args.warmup} warm-up); throughput does not depend on the flow state",
             "lattice_tile": main_m["tile"],
             "mlups_cells": "all cells incl. ghost layer (reference convention)",
             "interior_mlups": round(main_m["mlups"] * main_m["interior_cells"] / case.cells, 2),
@@ -436,7 +532,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(main_m["kernel_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(case.name),
+            "traffic": traffic,
+            "traffic_provenance": traffic_prov,
             "kernel": "k_lattice (fused computeForces+computeMoments+streamCollide)",
             "kernel_ms": round(main_m["kernel_ms"], 5),
             "algorithmic_bytes_per_launch": main_m["interior_cells"] * kb,
@@ -444,6 +541,11 @@ def main() -> int:
             "step_frac_at_own_step_bytes": round(main_m["step_frac_own_bytes"], 4),
             "device_copy_gbs": round(copy_gbs, 1),
             "frac_of_device_copy": round(main_m["kernel_gbs"] / copy_gbs, 4),
+            "ceiling_gbs": ceiling_gbs,
+            "frac_of_ceiling": round(main_m["kernel_gbs"] / ceiling_gbs, 4),
+            "ceiling": {k: {kk: v[kk] for kk in ("gbs", "nontemporal", "grid", "gib")} for k, v in best.items()},
+            "ceiling_note": "best rate of a hand-written dwordx4 stream (libssmd_probe.so) in this process; "
+                            "copy counts read + write bytes, as the lattice kernel's achieved rate does",
         },
         "cpu_baseline": cpu,
         "forms": forms,

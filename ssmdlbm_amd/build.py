@@ -21,6 +21,8 @@ DEPS = [CSRC / "lattice.hpp", CSRC / "kernels.hpp", CSRC / "launch.hpp", CSRC / 
 OBJDIR = LIBDIR / "obj"
 DRIVER_SRC = CSRC / "host" / "ssmd_run.cpp"
 DRIVER = LIBDIR / "ssmd_run"
+PROBE_SRC = CSRC / "probe" / "hbm_probe.hip"
+PROBE = LIBDIR / "libssmd_probe.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SSMD_OFFLOAD_ARCH", "gfx950")
@@ -87,6 +89,16 @@ def build_driver(verbose: bool = True) -> Path:
     return DRIVER
 
 
+def build_probe(verbose: bool = True) -> Path:
+    """Compile the HBM streaming probe (libssmd_probe.so) that bench.py measures the ceiling with."""
+    LIBDIR.mkdir(parents=True, exist_ok=True)
+    tmp = PROBE.with_suffix(".so.tmp")
+    _run([HIPCC, *HIP_FLAGS, "-shared", str(PROBE_SRC), "-o", str(tmp)], verbose)
+    os.replace(tmp, PROBE)
+    return PROBE
+
+
 if __name__ == "__main__":
     build_library()
     build_driver()
+    build_probe()
