@@ -247,6 +247,146 @@ __device__ __forceinline__ float closure_tau(const KParams &k, MP macp, const Ce
     return tau;
 }
 
+// Surface-tension force of one cell from the masked-normal ring
+// (phaseField.cuh:246-293): curvature 3 sum_i w_i c_i . n(x + c_i) over
+// opposite pairs grouped by weight class, F = -sigma kappa |grad phi-hat| n.
+// nzm / nz0 / nzp are the ring slots of planes z-1, z, z+1; ncen the cell's
+// index in a ring plane of row length NXl.  nc = (n, |grad phi-hat|) of the cell.
+template <int Q, int NXl, int NP>
+__device__ __forceinline__ void surface_force(const float (&s_n)[4][4][NP], int nzm, int nz0, int nzp, int ncen,
+                                              float sigma, float4 &nc, float &kx, float &ky, float &kz) {
+    nc = make_float4(s_n[nz0][0][ncen], s_n[nz0][1][ncen], s_n[nz0][2][ncen], s_n[nz0][3][ncen]);
+    // component a of the masked normal at offset (dx, dy, dz), addressed from
+    // the (-1, -1) corner of each plane so that every offset is a non-negative
+    // immediate of one LDS address per plane
+    const float *cm = &s_n[nzm][0][ncen - NXl - 1], *c0 = &s_n[nz0][0][ncen - NXl - 1],
+                *cp = &s_n[nzp][0][ncen - NXl - 1];
+    auto nat = [&](int a, int dx, int dy, int dz) {
+        const float *b = dz < 0 ? cm : (dz > 0 ? cp : c0);
+        return b[a * NP + (dy + 1) * NXl + dx + 1];
+    };
+    // the curvature needs only the sum over axes: one sum per weight class
+    Sum acc[3];
+    sfor<0, (Q - 1) / 2>([&](auto Pp) {
+        constexpr int i = 2 * Pp + 1;
+        constexpr Dir d = kDir[i];
+        constexpr int cls = i <= 6 ? 0 : (i <= 18 ? 1 : 2);
+        if constexpr (d.x != 0) {
+            const float a = nat(0, d.x, d.y, d.z), b = nat(0, -d.x, -d.y, -d.z);
+            acc[cls].add(d.x > 0 ? a - b : b - a);
+        }
+        if constexpr (d.y != 0) {
+            const float a = nat(1, d.x, d.y, d.z), b = nat(1, -d.x, -d.y, -d.z);
+            acc[cls].add(d.y > 0 ? a - b : b - a);
+        }
+        if constexpr (d.z != 0) {
+            const float a = nat(2, d.x, d.y, d.z), b = nat(2, -d.x, -d.y, -d.z);
+            acc[cls].add(d.z > 0 ? a - b : b - a);
+        }
+    });
+    // 3 w_class, the prefactor folded into the weights
+    constexpr float w1 = 3.0f * weight<Q>(1), w2 = 3.0f * weight<Q>(7), w3 = Q == 27 ? 3.0f * weight<Q>(19) : 0.f;
+    float curvature = w1 * acc[0].v + w2 * acc[1].v;
+    if constexpr (Q == 27) curvature += w3 * acc[2].v;
+    const float st = -sigma * curvature * nc.w;
+    kx = st * nc.x;
+    ky = st * nc.y;
+    kz = st * nc.z;
+}
+
+// Normal of the stored phi for the phase update (lbm.cu:157-224).  It equals
+// the masked normal nc unless the stencil reaches the ghost layer; there the
+// ghost values are added back, face by face (a neighbour on two ghost faces
+// is taken by the first).  Only for interior cells.
+template <int Q>
+__device__ __forceinline__ void stored_phi_normal(const KParams &k, const float *__restrict__ phi, int x, int y,
+                                                  int z, bool active, const float4 &nc, float &nx, float &ny,
+                                                  float &nz) {
+    nx = nc.x;
+    ny = nc.y;
+    nz = nc.z;
+    const bool fx_lo = x == 1, fx_hi = x == k.nx - 2, fy_lo = y == 1, fy_hi = y == k.ny - 2;
+    const bool fz_lo = z == 1, fz_hi = z == k.nz - 2;
+    if (!(active && (fx_lo || fx_hi || fy_lo || fy_hi || fz_lo || fz_hi))) return;
+    const float g = nc.w + 1e-9f;
+    float gx = nc.x * g, gy = nc.y * g, gz = nc.z * g;
+    // add 3 w_i c_i phi(x + c_i) for every neighbour on the ghost layer
+    auto face_sum = [&](auto axis_c, auto sign_c, bool on) {
+        constexpr int AX = decltype(axis_c)::value, SG = decltype(sign_c)::value;
+        if (!on) return;
+        sfor<1, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr Dir d = kDir[i];
+            constexpr int cv[3] = {d.x, d.y, d.z};
+            if constexpr (cv[AX] == SG) {
+                bool taken = false;
+                if constexpr (AX >= 1 && d.x != 0) taken = taken || (d.x < 0 ? fx_lo : fx_hi);
+                if constexpr (AX >= 2 && d.y != 0) taken = taken || (d.y < 0 ? fy_lo : fy_hi);
+                const float v = 3.0f * weight<Q>(i) * phi[cell_id(k, x + d.x, y + d.y, z + d.z)];
+                const float vv = taken ? 0.f : v;
+                if constexpr (d.x != 0) gx += float(d.x) * vv;
+                if constexpr (d.y != 0) gy += float(d.y) * vv;
+                if constexpr (d.z != 0) gz += float(d.z) * vv;
+            }
+        });
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using Pm = std::integral_constant<int, -1>;
+    using Pp = std::integral_constant<int, 1>;
+    face_sum(I0{}, Pm{}, fx_lo);
+    face_sum(I0{}, Pp{}, fx_hi);
+    face_sum(I1{}, Pm{}, fy_lo);
+    face_sum(I1{}, Pp{}, fy_hi);
+    face_sum(I2{}, Pm{}, fz_lo);
+    face_sum(I2{}, Pp{}, fz_hi);
+    const float ind = fast_sqrt(gx * gx + gy * gy + gz * gz);
+    const float inv = fast_rcp(ind + 1e-9f);
+    nx = gx * inv;
+    ny = gy * inv;
+    nz = gz * inv;
+}
+
+// Bytes of global memory, for pointers the compiler must not lose the
+// address space of when they pass through an opaque register copy.
+using GChar = __attribute__((address_space(1))) char;
+
+// Stores the Q post-collision populations of one cell, emitted in the order
+// 0, then opposite pairs (2k-1, 2k), each at its own plane (plane i at
+// plane0 + i * SB, advanced by one scalar add per plane) and byte offset
+// `so`, non-temporally: the next step reads them from HBM.  16-bit formats
+// round each pair with one packed conversion and store its halves.
+template <class P, class Collide>
+__device__ __forceinline__ void store_pops(uint32_t so, GChar *plane, size_t SB, Collide &&collide_cell) {
+    using PT = typename P::T;
+    using AS1 = __attribute__((address_space(1))) PT;
+    if constexpr (sizeof(PT) == 2) {
+        float first = 0.f;
+        GChar *first_plane = plane;
+        collide_cell([&](int i, float v) {
+            if (i > 0) plane += SB;
+            asm volatile("" : "+s"(plane));
+            if (i == 0) {
+                __builtin_nontemporal_store(P::store(v), reinterpret_cast<AS1 *>(plane + so));
+            } else if (i % 2 == 1) {
+                first = v;
+                first_plane = plane;
+            } else {
+                const uint32_t w = Pack2<P>::pack(first, v);
+                __builtin_nontemporal_store(PT(w), reinterpret_cast<AS1 *>(first_plane + so));
+                __builtin_nontemporal_store(PT(w >> 16), reinterpret_cast<AS1 *>(plane + so));
+            }
+        });
+    } else {
+        collide_cell([&](int i, float v) {
+            if (i > 0) plane += SB;
+            asm volatile("" : "+s"(plane));
+            __builtin_nontemporal_store(P::store(v), reinterpret_cast<AS1 *>(plane + so));
+        });
+    }
+}
+
 // Planes whose populations are in flight while a plane collides.  A 16-bit
 // pull moves half the bytes per instruction of an FP32 one; on D3Q27 a
 // second plane in flight restores them (+14% for the 16-bit kernels, still
@@ -254,9 +394,6 @@ __device__ __forceinline__ float closure_tau(const KParams &k, MP macp, const Ce
 // waves per SIMD) keep one.
 template <int Q, class P> constexpr int pull_depth() { return Q == 27 && sizeof(typename P::T) == 2 ? 2 : 1; }
 
-// Bytes of global memory, for pointers the compiler must not lose the
-// address space of when they pass through an opaque register copy.
-using GChar = __attribute__((address_space(1))) char;
 
 // One tile: cells x0..x0+TX-1, y0..y0+TY-1, planes zb..ze-1 (threadIdx = (tx, ty)).
 // TX is a multiple of 64, so a wavefront covers part of one x row: y is
@@ -340,14 +477,26 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             if (e < HP) s_m[sl][e] = r[q];
         }
     };
+    // normal-ring elements e = tid + q NT of this thread: the phi-hat index
+    // of their (-1, -1) stencil corner, the same on every plane
+    constexpr int NPT = (NP + NT - 1) / NT;
+    int ncorner[NPT];
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int e = tid + q * NT;
+        ncorner[q] = (e / NXl) * HX + e % NXl;  // (hy - 1) HX + hx - 1 with hx, hy = e % NXl + 1, e / NXl + 1
+    }
     auto normals = [&](int p) {
         const int sl = p % 4;
         const int pm = (p + 4) % 5, p0 = (p + 5) % 5, pp = (p + 6) % 5;
-        for (int e = tid; e < NP; e += NT) {
-            const int hx = e % NXl + 1, hy = e / NXl + 1;
+#pragma unroll
+        for (int q = 0; q < NPT; ++q) {
+            const int e = tid + q * NT;
+            if (q == NPT - 1 && e >= NP) break;
+            const float *cm = &s_m[pm][ncorner[q]], *c0 = &s_m[p0][ncorner[q]], *cp = &s_m[pp][ncorner[q]];
             auto at = [&](int dx, int dy, int dz) {
-                const int s = dz < 0 ? pm : (dz > 0 ? pp : p0);
-                return s_m[s][(hy + dy) * HX + hx + dx];
+                const float *b = dz < 0 ? cm : (dz > 0 ? cp : c0);
+                return b[(dy + 1) * HX + dx + 1];
             };
             float gx, gy, gz;
             iso_grad<Q>(at, gx, gy, gz);
@@ -446,102 +595,23 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
 
         // (3) surface-tension force from the masked normals (phaseField.cuh:246-293)
         // (2) everything below reads normal slots z-1..z+1 and phi slot z
-        const int nzm = (z + 3) % 4, nz0 = z % 4, nzp = (z + 1) % 4;
         const int ncen = (ty + 1) * NXl + tx + 1;
         float kx, ky, kz;
-        const float4 nc = make_float4(s_n[nz0][0][ncen], s_n[nz0][1][ncen], s_n[nz0][2][ncen], s_n[nz0][3][ncen]);
-        {
-            // component a of the masked normal at offset (dx, dy, dz)
-            auto nat = [&](int a, int dx, int dy, int dz) {
-                const int s = dz < 0 ? nzm : (dz > 0 ? nzp : nz0);
-                return s_n[s][a][ncen + dy * NXl + dx];
-            };
-            float acc[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-            sfor<0, (Q - 1) / 2>([&](auto Pp) {
-                constexpr int i = 2 * Pp + 1;
-                constexpr Dir d = kDir[i];
-                constexpr int cls = i <= 6 ? 0 : (i <= 18 ? 1 : 2);
-                if constexpr (d.x != 0) {
-                    const float a = nat(0, d.x, d.y, d.z), b = nat(0, -d.x, -d.y, -d.z);
-                    acc[cls][0] += d.x > 0 ? a - b : b - a;
-                }
-                if constexpr (d.y != 0) {
-                    const float a = nat(1, d.x, d.y, d.z), b = nat(1, -d.x, -d.y, -d.z);
-                    acc[cls][1] += d.y > 0 ? a - b : b - a;
-                }
-                if constexpr (d.z != 0) {
-                    const float a = nat(2, d.x, d.y, d.z), b = nat(2, -d.x, -d.y, -d.z);
-                    acc[cls][2] += d.z > 0 ? a - b : b - a;
-                }
-            });
-            constexpr float w1 = weight<Q>(1), w2 = weight<Q>(7), w3 = Q == 27 ? weight<Q>(19) : 0.f;
-            float sx = w1 * acc[0][0] + w2 * acc[1][0];
-            float sy = w1 * acc[0][1] + w2 * acc[1][1];
-            float sz = w1 * acc[0][2] + w2 * acc[1][2];
-            if constexpr (Q == 27) {
-                sx += w3 * acc[2][0];
-                sy += w3 * acc[2][1];
-                sz += w3 * acc[2][2];
-            }
-            const float curvature = 3.0f * (sx + sy + sz);
-            const float st = -k.sigma * curvature * nc.w;
-            kx = st * nc.x;
-            ky = st * nc.y;
-            kz = st * nc.z;
-        }
+        float4 nc;
+        surface_force<Q, NXl>(s_n, (z + 3) % 4, z % 4, (z + 1) % 4, ncen, k.sigma, nc, kx, ky, kz);
 
-        // (4) normal of the stored phi for the phase update (lbm.cu:157-224).
-        //     It equals the masked normal unless the stencil reaches the ghost
-        //     layer; there the ghost values are added back.
-        float nx = nc.x, ny = nc.y, nz = nc.z;
+        // (4) normal of the stored phi for the phase update (interior lanes
+        //     only: a ghost-column lane's x -+ 1 leaves the row)
+        float nx, ny, nz;
         const float ph = s_m[z % 5][(ty + 2) * HX + tx + 2];
-        const bool fx_lo = x == 1, fx_hi = x == k.nx - 2, fy_lo = y == 1, fy_hi = y == k.ny - 2;
-        const bool fz_lo = z == 1, fz_hi = z == k.nz - 2;
-        // (interior lanes only: a ghost-column lane's x -+ 1 leaves the row)
-        if (active && (fx_lo || fx_hi || fy_lo || fy_hi || fz_lo || fz_hi)) {
-            const float g = nc.w + 1e-9f;
-            float gx = nc.x * g, gy = nc.y * g, gz = nc.z * g;
-            // add 3 w_i c_i phi(x + c_i) for every neighbour on the ghost layer,
-            // face by face; a neighbour on two ghost faces is taken by the first
-            auto face_sum = [&](auto axis_c, auto sign_c, bool on) {
-                constexpr int AX = decltype(axis_c)::value, SG = decltype(sign_c)::value;
-                if (!on) return;
-                sfor<1, Q>([&](auto I) {
-                    constexpr int i = I;
-                    constexpr Dir d = kDir[i];
-                    constexpr int cv[3] = {d.x, d.y, d.z};
-                    if constexpr (cv[AX] == SG) {
-                        bool taken = false;
-                        if constexpr (AX >= 1 && d.x != 0) taken = taken || (d.x < 0 ? fx_lo : fx_hi);
-                        if constexpr (AX >= 2 && d.y != 0) taken = taken || (d.y < 0 ? fy_lo : fy_hi);
-                        const float v = 3.0f * weight<Q>(i) * phi[cell_id(k, x + d.x, y + d.y, z + d.z)];
-                        const float vv = taken ? 0.f : v;
-                        if constexpr (d.x != 0) gx += float(d.x) * vv;
-                        if constexpr (d.y != 0) gy += float(d.y) * vv;
-                        if constexpr (d.z != 0) gz += float(d.z) * vv;
-                    }
-                });
-            };
-            using I0 = std::integral_constant<int, 0>;
-            using I1 = std::integral_constant<int, 1>;
-            using I2 = std::integral_constant<int, 2>;
-            using Pm = std::integral_constant<int, -1>;
-            using Pp = std::integral_constant<int, 1>;
-            face_sum(I0{}, Pm{}, fx_lo);
-            face_sum(I0{}, Pp{}, fx_hi);
-            face_sum(I1{}, Pm{}, fy_lo);
-            face_sum(I1{}, Pp{}, fy_hi);
-            face_sum(I2{}, Pm{}, fz_lo);
-            face_sum(I2{}, Pp{}, fz_hi);
-            const float ind = fast_sqrt(gx * gx + gy * gy + gz * gz);
-            const float inv = fast_rcp(ind + 1e-9f);
-            nx = gx * inv;
-            ny = gy * inv;
-            nz = gz * inv;
-        }
+        stored_phi_normal<Q>(k, phi, x, y, z, active, nc, nx, ny, nz);
+        const bool fy_lo = y == 1, fy_hi = y == k.ny - 2, fz_lo = z == 1, fz_hi = z == k.nz - 2;
 
-        // (5) moments; then the pull of plane z+D; then the collision
-        const Cell m = moments<Q, P>(fr, kx, ky, kz);
+        // (5) moments; then the pull of plane z+D; then the collision.  A
+        //     ghost-column lane collides the rest state without force, which
+        //     emits exactly 0 for every population and phase moment.
+        if (!active) kx = ky = kz = 0.f;
+        const Cell m = rest_or(active, moments<Q, P>(fr, kx, ky, kz));
         if (z + D < ze) pull(z + D, fr);
         const float nu = __builtin_fmaf(ph, k.visc_diff, k.visc_water);
         const float omco = 1.0f - fast_rcp(k.closure == SSMD_CLOSURE_NONE
@@ -553,12 +623,7 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         const uint32_t so = c * uint32_t(sizeof(PT));
         GChar *splane = reinterpret_cast<GChar *>(reinterpret_cast<uintptr_t>(fout));
         const size_t SB = size_t(N0) * sizeof(PT);
-        collide<Q>(m, omco, kx, ky, kz, [&](int i, float v) {
-            if (i > 0) splane += SB;
-            asm volatile("" : "+s"(splane));
-            __builtin_nontemporal_store(P::store(active ? v : 0.f),
-                                        reinterpret_cast<__attribute__((address_space(1))) PT *>(splane + so));
-        });
+        store_pops<P>(so, splane, SB, [&](auto &&emit) { collide<Q>(m, omco, kx, ky, kz, emit); });
         const float sharp = k.gamma * ph * (1.0f - ph);
         const uint32_t go = c * 4u;
         GChar *gplane = reinterpret_cast<GChar *>(reinterpret_cast<uintptr_t>(gout));
@@ -568,9 +633,9 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             asm volatile("" : "+s"(gplane));
             auto *dst = reinterpret_cast<__attribute__((address_space(1))) float *>(gplane + go);
             if (k.g_cached)
-                *dst = active ? v : 0.f;
+                *dst = v;
             else
-                __builtin_nontemporal_store(active ? v : 0.f, dst);
+                __builtin_nontemporal_store(v, dst);
         });
 
         // rho, u, Pi on the planes the inlet / outflow kernels read them from

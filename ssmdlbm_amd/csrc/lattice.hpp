@@ -26,6 +26,22 @@ struct Dir {
     int x, y, z;
 };
 
+// A running sum whose first term is taken as is: 0 + x must be kept for
+// x = -0 in IEEE arithmetic, so starting from 0.f costs one add per sum.
+// `any` is known at compile time in the unrolled loops that use it.
+struct Sum {
+    float v = 0.f;
+    bool any = false;
+    __device__ __forceinline__ void add(float d) {
+        v = any ? v + d : d;
+        any = true;
+    }
+    __device__ __forceinline__ void sub(float d) {
+        v = any ? v - d : -d;
+        any = true;
+    }
+};
+
 // Shared table: D3Q19 is the first 19 entries, D3Q7 the first 7.
 constexpr Dir kDir[27] = {
     {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
@@ -94,29 +110,52 @@ struct PopBF16 {
     }
 };
 
+// Two post-collision populations of one cell rounded to a 16-bit storage
+// format in one instruction (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32,
+// round-to-nearest-even), returned as one word: the first in the low half.
+// The values are made opaque first, as P::store does, so that no FMA is
+// folded into the rounding.
+template <class P> struct Pack2;
+template <> struct Pack2<PopF16> {
+    typedef _Float16 H2 __attribute__((ext_vector_type(2)));
+    typedef float F2 __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ uint32_t pack(float a, float b) {
+        asm volatile("" : "+v"(a), "+v"(b));
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(F2{a, b}, H2));
+    }
+};
+template <> struct Pack2<PopBF16> {
+    typedef __bf16 B2 __attribute__((ext_vector_type(2)));
+    typedef float F2 __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ uint32_t pack(float a, float b) {
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(F2{a, b}, B2));
+    }
+};
+
 // --------------------------------------------------------------------------
 // Isotropic gradient 3 * sum_i w_i c_i v(x + c_i) over opposite pairs,
 // grouped by weight class (reference phaseField.cuh:158-218, lbm.cu:157-217).
 // `at(dx,dy,dz)` returns the stencil value.
 template <int Q, class At>
 __device__ __forceinline__ void iso_grad(At &&at, float &gx, float &gy, float &gz) {
-    float s[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};  // [class][axis]
+    Sum s[3][3];  // [class][axis]
     sfor<0, (Q - 1) / 2>([&](auto P) {
         constexpr int i = 2 * P + 1;
         constexpr Dir c = kDir[i];
         constexpr int cls = i <= 6 ? 0 : (i <= 18 ? 1 : 2);
         const float d = at(c.x, c.y, c.z) - at(-c.x, -c.y, -c.z);
-        if constexpr (c.x != 0) s[cls][0] += c.x > 0 ? d : -d;
-        if constexpr (c.y != 0) s[cls][1] += c.y > 0 ? d : -d;
-        if constexpr (c.z != 0) s[cls][2] += c.z > 0 ? d : -d;
+        if constexpr (c.x != 0) { if (c.x > 0) s[cls][0].add(d); else s[cls][0].sub(d); }
+        if constexpr (c.y != 0) { if (c.y > 0) s[cls][1].add(d); else s[cls][1].sub(d); }
+        if constexpr (c.z != 0) { if (c.z > 0) s[cls][2].add(d); else s[cls][2].sub(d); }
     });
-    constexpr float w1 = weight<Q>(1), w2 = weight<Q>(7), w3 = Q == 27 ? weight<Q>(19) : 0.f;
+    // 3 w_class, the prefactor folded into the weights
+    constexpr float w1 = 3.0f * weight<Q>(1), w2 = 3.0f * weight<Q>(7), w3 = Q == 27 ? 3.0f * weight<Q>(19) : 0.f;
     float r[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float v = w1 * s[0][a] + w2 * s[1][a];
-        if constexpr (Q == 27) v += w3 * s[2][a];
-        r[a] = 3.0f * v;
+        float v = w1 * s[0][a].v + w2 * s[1][a].v;
+        if constexpr (Q == 27) v += w3 * s[2][a].v;
+        r[a] = v;
     }
     gx = r[0];
     gy = r[1];
@@ -140,35 +179,58 @@ struct Cell {
     float pxx, pyy, pzz, pxy, pxz, pyz;
 };
 
-template <int Q, class P>
-__device__ __forceinline__ Cell moments(const typename P::T (&f)[Q], float fx, float fy, float fz) {
+// The rest state rho = 1, u = 0, Pi = 0: with F = 0 every population and
+// phase moment collide() and phase_moments() emit from it is exactly +0.
+__device__ __forceinline__ Cell rest_or(bool keep, const Cell &m) {
+    Cell r;
+    r.rho = keep ? m.rho : 1.0f;
+    r.drho = keep ? m.drho : 0.f;
+    r.ux = keep ? m.ux : 0.f;
+    r.uy = keep ? m.uy : 0.f;
+    r.uz = keep ? m.uz : 0.f;
+    r.pxx = keep ? m.pxx : 0.f;
+    r.pyy = keep ? m.pyy : 0.f;
+    r.pzz = keep ? m.pzz : 0.f;
+    r.pxy = keep ? m.pxy : 0.f;
+    r.pxz = keep ? m.pxz : 0.f;
+    r.pyz = keep ? m.pyz : 0.f;
+    return r;
+}
+
+// `get(i)` returns the stored element of direction i for the cell.
+template <int Q, class P, class Get>
+__device__ __forceinline__ Cell moments_of(Get &&get, float fx, float fy, float fz) {
     Cell m;
     float one = 1.0f;
     asm volatile("" : "+v"(one));  // not folded: P::pair multiplies by it
-    float dr = P::load(f[0]);
-    float jx = 0.f, jy = 0.f, jz = 0.f;
-    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+    float dr = P::load(get(0));
+    Sum jx, jy, jz, sxx, syy, szz, sxy, sxz, syz;
     sfor<0, (Q - 1) / 2>([&](auto Pp) {
         constexpr int i = 2 * Pp + 1;
         constexpr Dir c = kDir[i];
         float sum, dif;
-        P::pair(f[i], f[i + 1], one, sum, dif);
+        P::pair(get(i), get(i + 1), one, sum, dif);
         dr += sum;
-        if constexpr (c.x != 0) { jx += c.x > 0 ? dif : -dif; sxx += sum; }
-        if constexpr (c.y != 0) { jy += c.y > 0 ? dif : -dif; syy += sum; }
-        if constexpr (c.z != 0) { jz += c.z > 0 ? dif : -dif; szz += sum; }
-        if constexpr (c.x * c.y != 0) sxy += c.x * c.y > 0 ? sum : -sum;
-        if constexpr (c.x * c.z != 0) sxz += c.x * c.z > 0 ? sum : -sum;
-        if constexpr (c.y * c.z != 0) syz += c.y * c.z > 0 ? sum : -sum;
+        auto signed_add = [](Sum &a, int sg, float v) {
+            if (sg > 0) a.add(v);
+            else a.sub(v);
+        };
+        if constexpr (c.x != 0) { signed_add(jx, c.x, dif); sxx.add(sum); }
+        if constexpr (c.y != 0) { signed_add(jy, c.y, dif); syy.add(sum); }
+        if constexpr (c.z != 0) { signed_add(jz, c.z, dif); szz.add(sum); }
+        if constexpr (c.x * c.y != 0) signed_add(sxy, c.x * c.y, sum);
+        if constexpr (c.x * c.z != 0) signed_add(sxz, c.x * c.z, sum);
+        if constexpr (c.y * c.z != 0) signed_add(syz, c.y * c.z, sum);
     });
     const float rho = dr + 1.0f;
-    const float inv = 1.0f / rho;
-    const float ux = inv * jx + fx * 0.5f * inv;
-    const float uy = inv * jy + fy * 0.5f * inv;
-    const float uz = inv * jz + fz * 0.5f * inv;
-    float pxx = sxx - rho * ux * ux + ux * fx;
-    float pyy = syy - rho * uy * uy + uy * fy;
-    float pzz = szz - rho * uz * uz + uz * fz;
+    // one-ulp reciprocal: u, Pi stay far inside the FP32 budget of the step
+    const float inv = __builtin_amdgcn_rcpf(rho);
+    const float ux = inv * jx.v + fx * 0.5f * inv;
+    const float uy = inv * jy.v + fy * 0.5f * inv;
+    const float uz = inv * jz.v + fz * 0.5f * inv;
+    float pxx = sxx.v - rho * ux * ux + ux * fx;
+    float pyy = syy.v - rho * uy * uy + uy * fy;
+    float pzz = szz.v - rho * uz * uz + uz * fz;
     const float tr = pxx + pyy + pzz;
     pxx -= kCs2 * tr;
     pyy -= kCs2 * tr;
@@ -181,10 +243,15 @@ __device__ __forceinline__ Cell moments(const typename P::T (&f)[Q], float fx, f
     m.pxx = pxx;
     m.pyy = pyy;
     m.pzz = pzz;
-    m.pxy = sxy - rho * ux * uy + 0.5f * (ux * fy + uy * fx);
-    m.pxz = sxz - rho * ux * uz + 0.5f * (ux * fz + uz * fx);
-    m.pyz = syz - rho * uy * uz + 0.5f * (uy * fz + uz * fy);
+    m.pxy = sxy.v - rho * ux * uy + 0.5f * (ux * fy + uy * fx);
+    m.pxz = sxz.v - rho * ux * uz + 0.5f * (ux * fz + uz * fx);
+    m.pyz = syz.v - rho * uy * uz + 0.5f * (uy * fz + uz * fy);
     return m;
+}
+
+template <int Q, class P>
+__device__ __forceinline__ Cell moments(const typename P::T (&f)[Q], float fx, float fy, float fz) {
+    return moments_of<Q, P>([&](int i) { return f[i]; }, fx, fy, fz);
 }
 
 // Emits w_i * (even +- odd) for every direction through `emit(i, value)`.

@@ -98,12 +98,14 @@ def test_seeded_32_jet_ensemble_at_its_own_boxes(gpu):
     ens.close()
 
 
-@pytest.mark.parametrize("q,pop", [(19, abi.POP_F32), (27, abi.POP_F32), (27, abi.POP_F16)])
-def test_tile_shapes_agree(gpu, q, pop):
-    # two x tiles of 128 (one partial), partial y tiles for every shape; for
-    # the 16-bit D3Q27 kernels (two planes of pulls in flight, the z loop
-    # unrolled by two) also z chunks of odd length and a partial last chunk
-    c = cases.JET_D3Q19_SMALL.with_(nx=136, ny=30, nz=40, q=q, diameter=8, pop_format=pop)
+@pytest.mark.parametrize("q,pop,nx", [(19, abi.POP_F32, 136), (27, abi.POP_F32, 136), (27, abi.POP_F16, 136),
+                                      (19, abi.POP_F16, 136), (19, abi.POP_BF16, 136), (19, abi.POP_F16, 135)])
+def test_tile_shapes_agree(gpu, q, pop, nx):
+    # two x tiles of 128 (one partial; an odd row length too), partial y
+    # tiles for every shape; for 16-bit storage also z chunks of odd length
+    # and a partial last chunk (the D3Q27 16-bit kernels keep two planes of
+    # pulls in flight, the z loop unrolled by two)
+    c = cases.JET_D3Q19_SMALL.with_(nx=nx, ny=30, nz=40, q=q, diameter=8, pop_format=pop)
     lib = abi.load()
     res = []
     runs = [(t, 0) for t in range(lib.ssmd_tile_count())]
