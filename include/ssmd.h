@@ -103,7 +103,8 @@ typedef struct ssmd_params {
     int32_t outflow_z;   /* applyOutflowZ on z = nz-1 */
     int32_t wrap_x, wrap_y, wrap_z; /* SSMD_WRAP_* */
     int32_t tile;        /* lattice tile shape: index into the built shapes
-                            (0: 64x4, 1: 128x2, 2: 64x2, 3: 128x4); see ssmd_tune */
+                            (0: 64x4, 1: 128x2, 2: 64x2, 3: 128x4; 4: 128x4 with the
+                            tiles dealt to the XCDs in contiguous ranges); see ssmd_tune */
     int32_t lz;          /* z planes per lattice block; 0 = automatic */
     double visc_water;   /* relaxation::visc_water() */
     double visc_oil;     /* relaxation::visc_oil() */

@@ -106,6 +106,9 @@ __device__ __forceinline__ void phase_keep(float *__restrict__ phi, uint32_t c) 
 // over kPhaseLZ planes; tiles numbered x-block fastest, then y-block, then
 // z-chunk, one block per tile: ceil(nx/64) ceil((ny-2)/4) ceil((nz-2)/kPhaseLZ).
 constexpr int kPhaseLZ = 4;
+// Planes per block of k_phase_quad (28 us instead of 32 at 4 planes on a
+// 128 x 128 x 512 box; equal on 256 x 256 x 1024).
+constexpr int kPhaseQuadLZ = 8;
 
 // K1: phi = sum of the seven arriving phase populations (phaseField.cuh:31-52).
 //
@@ -119,6 +122,21 @@ constexpr int kPhaseLZ = 4;
 // the collision used (an add, then a power-of-two scale), so phi is the sum of
 // the same seven FP32 values, in the same order, as with population storage.
 //
+// phi of one cell from its seven arriving populations, summed in order:
+// rest, +x (source x-1), -x (x+1), +y, -y, +z, -z, each formed as
+// w_k (g_phi + c_k . o) of its source cell.
+__device__ __forceinline__ float phase_sum(float cg, float g1, float o1, float g2, float o2, float g3, float o3,
+                                           float g4, float o4, float bg, float bz, float ag, float az) {
+    float s = weight7(0) * cg;
+    s += weight7(1) * (g1 + o1);
+    s += weight7(2) * (g2 - o2);
+    s += weight7(3) * (g3 + o3);
+    s += weight7(4) * (g4 - o4);
+    s += weight7(5) * (bg + bz);
+    s += weight7(6) * (ag - az);
+    return s;
+}
+
 // One column (x, y), planes z0..z1-1: the +z and -z sources of a plane are
 // the own column's values of the planes below and above, carried in
 // registers from plane to plane (each plane loads only the one above).
@@ -152,14 +170,7 @@ __device__ __forceinline__ void phase_column(const KParams &k, const float *__re
         const float ag = gp[sa], az = oz[sa];  // source of the -z population (plane above)
         const uint32_t s1 = uint32_t(ci - 1 + sxp), s2 = uint32_t(ci + 1 + sxm);
         const uint32_t s3 = uint32_t(ci - k.nx + syp), s4 = uint32_t(ci + k.nx + sym);
-        float s = weight7(0) * cg;
-        s += weight7(1) * (gp[s1] + ox[s1]);
-        s += weight7(2) * (gp[s2] - ox[s2]);
-        s += weight7(3) * (gp[s3] + oy[s3]);
-        s += weight7(4) * (gp[s4] - oy[s4]);
-        s += weight7(5) * (bg + bz);
-        s += weight7(6) * (ag - az);
-        phi[ci] = s;
+        phi[ci] = phase_sum(cg, gp[s1], ox[s1], gp[s2], ox[s2], gp[s3], oy[s3], gp[s4], oy[s4], bg, bz, ag, az);
         bg = cg;
         bz = cz;
         cg = ag;
@@ -183,6 +194,88 @@ __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restric
         return;
     }
     phase_column(k, g, phi, x, y, z0, z1);
+}
+
+// The phase kernel for rows of a multiple of 4 cells (every plane then starts
+// on a 16-byte boundary): a lane takes the 4 x-adjacent cells of one 16-byte
+// word of each moment plane and a block 256 consecutive such quads of the
+// interior rows (x fastest), marching over kPhaseQuadLZ planes as
+// phase_column does, with blocks dealt to the XCDs in contiguous ranges.  Per plane a lane
+// loads five 16-byte words (g_phi, o_x of its row; g_phi, o_y of the rows
+// y -+ 1; g_phi, o_z of the plane above -- the planes below and its own are
+// carried) and the two 4-byte ends x-1, x+4 of the g_phi and o_x rows, and
+// stores one 16-byte word of phi: a quarter of the memory instructions of
+// k_phase, every cell summed by phase_sum as there.  Ghost-column cells
+// (x = 0, nx-1) keep the phi they hold.
+__global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__restrict__ g,
+                                                    float *__restrict__ phi) {
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const uint32_t qx = uint32_t(k.nx) >> 2, rowq = qx * uint32_t(k.ny - 2), pb = (rowq + 255u) >> 8;
+    const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
+    const uint32_t zc = t / pb, q = (t - zc * pb) * 256u + threadIdx.x;
+    if (q >= rowq) return;
+    const uint32_t yq = q / qx;
+    const int x = int(q - yq * qx) * 4, y = 1 + int(yq);
+    const int z0 = 1 + int(zc) * kPhaseQuadLZ, z1 = min(z0 + kPhaseQuadLZ, k.nz - 1);
+    const bool first = x == 0, last = x == k.nx - 4;  // quads holding a ghost column
+    // periodic images: the +x source of cell 1 (first quad) and the -x source
+    // of cell nx-2 (last quad), the y rows of y = 1 / ny-2, the z planes
+    int sxp = 0, sxm = 0, syp = 0, sym = 0;
+    if (k.wrap_x) {
+        const int sh = k.wrap_x == SSMD_WRAP_TRUE ? k.nx - 2 : k.nx - 3;
+        sxp = sh;
+        sxm = -sh;
+    }
+    if (k.wrap_y) {
+        const int sh = (k.wrap_y == SSMD_WRAP_TRUE ? k.ny - 2 : k.ny - 3) * k.nx;
+        if (y == 1) syp = sh;
+        if (y == k.ny - 2) sym = -sh;
+    }
+    const int zsh = k.wrap_z ? (k.wrap_z == SSMD_WRAP_TRUE ? k.nz - 2 : k.nz - 3) * int(k.sxy) : 0;
+    const size_t N = k.N;
+    const float *__restrict__ gp = g;
+    const float *__restrict__ ox = g + N;
+    const float *__restrict__ oy = g + 2 * N;
+    const float *__restrict__ oz = g + 3 * N;
+    const int sxy = int(k.sxy);
+    auto ld4 = [](const float *base, int i) { return *reinterpret_cast<const F4 *>(base + i); };
+    int ci = int(cell_id(k, x, y, z0));
+    const int sb = ci - sxy + (z0 == 1 ? zsh : 0);
+    F4 bg = ld4(gp, sb), bz = ld4(oz, sb);  // sources of the +z populations (plane below)
+    F4 cg = ld4(gp, ci), cz = ld4(oz, ci);  // own cells, this plane
+    for (int z = z0; z < z1; ++z, ci += sxy) {
+        const int sa = ci + sxy - (z == k.nz - 2 ? zsh : 0);
+        const F4 ag = ld4(gp, sa), az = ld4(oz, sa);  // sources of the -z populations (plane above)
+        const F4 xo = ld4(ox, ci);
+        // x sources: elements x-1 .. x+4 of the g_phi and o_x rows
+        float G[6] = {gp[ci - 1], cg.x, cg.y, cg.z, cg.w, gp[ci + 4]};
+        float O[6] = {ox[ci - 1], xo.x, xo.y, xo.z, xo.w, ox[ci + 4]};
+        if (k.wrap_x && first) {  // cell 1 pulls +x from the image of x = 0
+            G[1] = gp[ci + sxp];
+            O[1] = ox[ci + sxp];
+        }
+        if (k.wrap_x && last) {  // cell nx-2 pulls -x from the image of x = nx-1
+            G[4] = gp[ci + 3 + sxm];
+            O[4] = ox[ci + 3 + sxm];
+        }
+        const int r3 = ci - k.nx + syp, r4 = ci + k.nx + sym;
+        const F4 g3 = ld4(gp, r3), o3 = ld4(oy, r3), g4 = ld4(gp, r4), o4 = ld4(oy, r4);
+        F4 out;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            out[e] = phase_sum(cg[e], G[e], O[e], G[e + 2], O[e + 2], g3[e], o3[e], g4[e], o4[e], bg[e], bz[e],
+                               ag[e], az[e]);
+        if (first || last) {  // the ghost column's phi is boundary data: kept
+            const F4 old = ld4(phi, ci);
+            if (first) out.x = old.x;
+            if (last) out.w = old.w;
+        }
+        *reinterpret_cast<F4 *>(phi + ci) = out;
+        bg = cg;
+        bz = cz;
+        cg = ag;
+        cz = az;
+    }
 }
 #endif  // SSMD_STEP_KERNELS
 
@@ -683,6 +776,12 @@ struct LatticeArgs {
     const float *phi;
     Macro mac;
     int lz;
+    // 0: block (bx, by, bz) takes tile (bx, by, bz), so consecutive tiles of
+    // a row land on different XCDs (blocks are dealt to the 8 XCDs in turn);
+    // 1: the tiles, numbered x fastest, then y, then z, are dealt to the XCDs
+    // in contiguous ranges, so x- and y-neighbour tiles -- and the cache lines
+    // their misaligned pulls share -- sit behind one L2
+    int xcd_order;
 };
 
 template <int Q, class P, int TX, int TY, bool MACROS>
@@ -691,10 +790,17 @@ __global__ void __launch_bounds__(TX *TY) k_lattice(LatticeArgs a) {
     using PT = typename P::T;
     const auto *seg = reinterpret_cast<const __attribute__((address_space(4))) LatticeArgs *>(
         reinterpret_cast<uintptr_t>(__builtin_amdgcn_kernarg_segment_ptr()));
-    const int zb = 1 + int(blockIdx.z) * a.lz;
+    uint32_t bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (a.xcd_order) {
+        const uint32_t gxy = gridDim.x * gridDim.y;
+        const uint32_t t = xcd_contiguous(bx + gridDim.x * (by + gridDim.y * bz), gxy * gridDim.z);
+        bz = t / gxy;
+        by = (t - bz * gxy) / gridDim.x;
+        bx = t - bz * gxy - by * gridDim.x;
+    }
+    const int zb = 1 + int(bz) * a.lz;
     lattice_tile<Q, P, TX, TY, MACROS>(a.k, static_cast<const PT *>(a.fin), static_cast<PT *>(a.fout), a.gout, a.phi,
-                                       &seg->mac, int(blockIdx.x) * TX, int(blockIdx.y) * TY, zb,
-                                       min(zb + a.lz, a.k.nz - 1), sm);
+                                       &seg->mac, int(bx) * TX, int(by) * TY, zb, min(zb + a.lz, a.k.nz - 1), sm);
 }
 
 // --------------------------------------------------------------------------

@@ -45,6 +45,7 @@ hipError_t launch_lattice(int tile, bool macros, const LatticeArgs &a, hipStream
         case 1: return detail::lattice_one<Q, P, 128, 2, false>(a, s);
         case 2: return detail::lattice_one<Q, P, 64, 2, false>(a, s);
         case 3: return detail::lattice_one<Q, P, 128, 4, false>(a, s);
+        case 4: return detail::lattice_one<Q, P, 128, 4, false>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -58,6 +59,9 @@ hipError_t launch_lattice_batch(int tile, bool macros, const JetSlot *jets, int 
         case 1: return detail::lattice_batch_one<Q, P, 128, 2, false>(jets, njets, total_tiles, queue, s);
         case 2: return detail::lattice_batch_one<Q, P, 64, 2, false>(jets, njets, total_tiles, queue, s);
         case 3: return detail::lattice_batch_one<Q, P, 128, 4, false>(jets, njets, total_tiles, queue, s);
+        // the persistent kernel takes tiles from a queue: the XCD order of
+        // tile 4 does not apply, its shape does
+        case 4: return detail::lattice_batch_one<Q, P, 128, 4, false>(jets, njets, total_tiles, queue, s);
         default: return hipErrorInvalidValue;
     }
 }

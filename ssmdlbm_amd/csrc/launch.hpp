@@ -7,11 +7,13 @@
 
 namespace ssmd {
 
-// Tile shapes the lattice kernels are built for (the autotuner's candidates).
+// Tile shapes the lattice kernels are built for (the autotuner's candidates),
+// and the order tiles are dealt to the XCDs in (LatticeArgs::xcd_order).
 struct TileShape {
     int tx, ty;
+    int xcd_order;
 };
-constexpr TileShape kTiles[] = {{64, 4}, {128, 2}, {64, 2}, {128, 4}};
+constexpr TileShape kTiles[] = {{64, 4, 0}, {128, 2, 0}, {64, 2, 0}, {128, 4, 0}, {128, 4, 1}};
 constexpr int kNumTiles = int(sizeof(kTiles) / sizeof(kTiles[0]));
 
 // z planes per block: enough chunks that the grid holds >= ~2048 blocks on

@@ -167,10 +167,17 @@ int face_axis_host(int kind) { return face_axis(kind); }
 
 int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
     const KParams k = make_kparams(p);
-    // one block per 64 x 4 tile of interior rows and kPhaseLZ planes (the kernel derives the tile)
-    const unsigned tiles = (unsigned(p.nx + 63) / 64) * (unsigned(p.ny - 2 + 3) / 4) *
-                           unsigned((p.nz - 2 + kPhaseLZ - 1) / kPhaseLZ);
-    hipLaunchKernelGGL(k_phase, dim3(tiles), dim3(64, 4), 0, s, k, d.g, d.phi);
+    if (p.nx % 4 == 0) {
+        // one block per 256 quads of interior rows and kPhaseQuadLZ planes
+        const unsigned quads = (p.nx / 4) * (p.ny - 2);
+        const unsigned chunks = unsigned((p.nz - 2 + kPhaseQuadLZ - 1) / kPhaseQuadLZ);
+        hipLaunchKernelGGL(k_phase_quad, dim3(((quads + 255) / 256) * chunks), dim3(256), 0, s, k, d.g, d.phi);
+    } else {
+        const unsigned chunks = unsigned((p.nz - 2 + kPhaseLZ - 1) / kPhaseLZ);
+        // one block per 64 x 4 tile of interior rows and kPhaseLZ planes (the kernel derives the tile)
+        const unsigned tiles = (unsigned(p.nx + 63) / 64) * (unsigned(p.ny - 2 + 3) / 4) * chunks;
+        hipLaunchKernelGGL(k_phase, dim3(tiles), dim3(64, 4), 0, s, k, d.g, d.phi);
+    }
     return herr(hipGetLastError());
 }
 
@@ -183,6 +190,7 @@ LatticeArgs lattice_args(const ssmd_params &p, const ssmd_fields &d, int lz) {
     a.phi = d.phi;
     a.mac = make_macro(d);
     a.lz = lz;
+    a.xcd_order = 0;
     return a;
 }
 
@@ -190,7 +198,8 @@ int launch_lattice_sel(const ssmd_params &p, const ssmd_fields &d, uint32_t flag
     const bool macros = (flags & SSMD_STEP_MACROS) != 0;
     // the macro-writing kernel is built for tile 0 only; per-cell results do not depend on the tile
     const int t = macros ? 0 : tile;
-    const LatticeArgs a = lattice_args(p, d, lz > 0 ? lz : auto_lz(int(p.nx), int(p.ny), int(p.nz), t));
+    LatticeArgs a = lattice_args(p, d, lz > 0 ? lz : auto_lz(int(p.nx), int(p.ny), int(p.nz), t));
+    a.xcd_order = kTiles[t].xcd_order;
     return dispatch(p, [&]<int Q, class P>() { return herr(launch_lattice<Q, P>(t, macros, a, s)); });
 }
 

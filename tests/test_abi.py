@@ -128,7 +128,7 @@ def test_tile_shapes():
         tx, ty = C.c_int32(), C.c_int32()
         assert lib.ssmd_tile_shape(t, C.byref(tx), C.byref(ty)) == 0
         shapes.append((tx.value, ty.value))
-    assert shapes == [(64, 4), (128, 2), (64, 2), (128, 4)]
+    assert shapes == [(64, 4), (128, 2), (64, 2), (128, 4), (128, 4)]
     for case in (cases.JET_D3Q19_SMALL, cases.JET_D3Q27):
         for t in range(len(shapes)):
             b = case.params().copy()
