@@ -170,9 +170,12 @@ class Stats:
         self.ay = [int(np.floor(cy)), int(np.ceil(cy))]
         self.z0, self.z1 = z_range
         self.kmax = kmax
-        n = self.nx - 2
-        k = torch.fft.fftfreq(n, d=1.0 / n, device=device)
-        kr = torch.sqrt(k[None, :] ** 2 + k[:, None] ** 2).round().long()
+        # integer mode numbers of the (ny-2) x (nx-2) interior slice, binned
+        # by rounded radius; the axes may differ in length
+        mx, my = self.nx - 2, self.ny - 2
+        kx = torch.fft.fftfreq(mx, d=1.0 / mx, device=device)
+        ky = torch.fft.fftfreq(my, d=1.0 / my, device=device)
+        kr = torch.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2).round().long()
         self.bins = kr
         self.centre = torch.zeros(int(p.nz), dtype=torch.float64, device=device)
         self.spec = torch.zeros(kmax + 1, dtype=torch.float64, device=device)
