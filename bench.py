@@ -360,20 +360,23 @@ def copy_bandwidth(reps: int = 10) -> float:
     return gbs
 
 
-def profile_mode(case, steps: int, tune: bool) -> int:
-    """Tuned like the timed run, then `steps` eager steps (one dispatch per
-    kernel, as counter collection wants them), then torch copies of 2^28
-    floats = 1 GiB read + 1 GiB written (the byte-count reference)."""
+def profile_mode(case, steps: int, tune: bool, tile=None) -> int:
+    """Tuned like the timed run (or with the given (tile, lz)), then `steps`
+    eager steps (one dispatch per kernel, as counter collection wants them),
+    then torch copies of 2^28 floats = 1 GiB read + 1 GiB written (the
+    byte-count reference)."""
     import torch
 
     from ssmdlbm_amd.solver import Simulation
 
     sim = Simulation(case.params(), "cuda:0")
     sim.init()
-    tile = (int(sim.p.tile), int(sim.p.lz))
-    if tune:
+    if tile is not None:
+        sim.p.tile, sim.p.lz = tile
+    elif tune:
         sim.step(2)
         tile = sim.tune(reps=10)
+    tile = (int(sim.p.tile), int(sim.p.lz))
     sim.step(steps)
     sim.synchronize()
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
@@ -437,6 +440,8 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling mode for rocprofv3 --pmc: N eager steps of the workload, then a plain "
                          "1 GiB device copy (the byte-count reference), no timing output")
+    ap.add_argument("--tile", type=int, nargs=2, metavar=("TILE", "LZ"),
+                    help="profiling mode: run this lattice tile shape and z-chunk instead of tuning")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -456,7 +461,7 @@ def main() -> int:
     if args.hbm_probe:
         return hbm_probe_mode((1, 4))
     if args.profile_steps:
-        return profile_mode(case, args.profile_steps, tune=not args.no_tune)
+        return profile_mode(case, args.profile_steps, tune=not args.no_tune, tile=args.tile)
     log(f"workload {case.name}: warmup {args.warmup}, timed {args.steps}")
     main_m = measure(case, args.steps, args.warmup, args.kernel_steps, tune=not args.no_tune)
     log(f"main: {main_m}")

@@ -213,11 +213,13 @@ int ssmd_swap(ssmd_fields *d);
 /* nsteps calls of ssmd_step for steps step0 .. step0+nsteps-1. */
 int ssmd_run(const ssmd_params *p, ssmd_fields *d, uint64_t step0, uint64_t nsteps, void *stream);
 
-/* Capture the step (both buffer parities) into HIP graphs.  The boundary
- * kernels are inside the graph; the inlet-noise step index comes from
- * d->step_counter, advanced on the device by the last kernel of each step. */
+/* Capture the step (both buffer parities) into HIP graphs, once as a single
+ * step and once as a block of 16 steps.  The boundary kernels are inside the
+ * graphs; the inlet-noise step index comes from d->step_counter, advanced on
+ * the device by the last kernel of each step. */
 int ssmd_graph_create(const ssmd_params *p, const ssmd_fields *d, void *stream, ssmd_graph **out);
-/* Replay nsteps steps starting at step index step0; swaps *d as ssmd_step does. */
+/* Replay nsteps steps starting at step index step0 (whole 16-step blocks,
+ * then single steps); swaps *d as ssmd_step does. */
 int ssmd_graph_launch(ssmd_graph *g, ssmd_fields *d, uint64_t step0, uint64_t nsteps, void *stream);
 int ssmd_graph_destroy(ssmd_graph *g);
 
