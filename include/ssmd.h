@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SSMD_ABI_VERSION 5
+#define SSMD_ABI_VERSION 6
 
 #define SSMD_OK 0
 #define SSMD_EINVAL (-1)       /* bad argument or inconsistent parameters */
@@ -129,6 +129,14 @@ typedef struct ssmd_params {
     double grad_scale;
     double tau_scale;
     double tau_floor;
+    /* LEARNED: subgrid flux of the phase field as gradient diffusion with the
+     * eddy viscosity, J = -D_t grad(phi), D_t = max(flux_coeff tau_t / 3, -0.1)
+     * (flux_coeff = 1/Sc_t, an inverse turbulent Schmidt number): the D3Q7
+     * first moments of the step get -4 D_t grad(phi), i.e. the sharpening
+     * coefficient gamma phi (1 - phi) becomes gamma phi (1 - phi) - 4 D_t
+     * |grad phi| along the normal.  0 (default): no flux closure; nonzero
+     * values require the LEARNED closure (else SSMD_EINVAL). */
+    double flux_coeff;
 } ssmd_params;
 
 /* Device buffers.  Member order follows struct LBMFields (LBMFields.cuh:26-85);
@@ -354,14 +362,15 @@ int ssmd_mlp_adam(float *params, const float *grad, float *m, float *v, uint32_t
  * step wrote every moment (SSMD_STEP_MACROS), box-filtered by 2 onto the
  * coarse grid `coarse` (coarse interior cell i covers fine cells 2i-1 and 2i
  * per axis; requires 2 (n_c - 2) <= n_f - 2 on each axis).
- * work receives 17 coarse planes (ssmd_sgs_work_bytes): rho, Favre-filtered
+ * work receives 20 coarse planes (ssmd_sgs_work_bytes): rho, Favre-filtered
  * ux uy uz, phi, the filtered strain rate S (xx yy zz xy xz yz, per coarse
- * step) and the subgrid stress T = <rho u u> - <rho> u~ u~ (same order),
- * all in coarse lattice units.  The two runs must describe one flow: with
+ * step), the subgrid stress T = <rho u u> - <rho> u~ u~ (same order) and
+ * the subgrid flux of the phase field J = <phi u> - <phi> <u> (x y z; the
+ * target of the flux_coeff closure), all in coarse lattice units.  The two runs must describe one flow: with
  * dx_c = 2 dx_f, their water viscosities give the time ratio rt = dt_c / dt_f
  * = 4 nu_c / nu_f (2 for acoustic scaling, nu_f = 2 nu_c; 4 for diffusive
  * scaling, nu_f = nu_c; 1 <= rt <= 16 or SSMD_EINVAL), and fine velocities,
- * strain rates and stresses are scaled by rt / 2, rt and (rt / 2)^2.
+ * strain rates, stresses and fluxes are scaled by rt / 2, rt, (rt / 2)^2 and rt / 2.
  * Per coarse cell: feat = the 8 network inputs as the LEARNED step forms them
  * (coarse->strain_scale, grad_scale), target = 3 nu_t / coarse->tau_scale with
  * nu_t = -(dev T : S) / (2 rho S:S) the least-squares eddy viscosity, and

@@ -253,6 +253,7 @@ struct Oracle final : Base {
     std::vector<T> f, g;
     std::vector<T> nnx, nny, nnz, nind;  // K2 normals (tile storage in the reference)
     T visc_water, visc_oil, visc_diff, omco_water, omco_oil, omco_ref, sigma, gamma;
+    T flux3;  // LEARNED flux closure: flux_coeff / 3, formed in double
 
     enum { RHO, UX, UY, UZ, PXX, PYY, PZZ, PXY, PXZ, PYZ, PHI, FSX, FSY, FSZ, TAU_T, FEAT0, NFIELDS = FEAT0 + 8 };
 
@@ -276,6 +277,7 @@ struct Oracle final : Base {
         omco_ref = omco_water;
         sigma = T(p.sigma);
         gamma = T(p.gamma);
+        flux3 = T(p.flux_coeff / 3.0);
     }
 
     void set_threads(int n) override {
@@ -514,7 +516,12 @@ struct Oracle final : Base {
                         const size_t dst = id3(x + C27[i][0], y + C27[i][1], z + C27[i][2]);
                         f[size_t(i) * N + dst] = store(feq + omco * fneq + T(0.5) * force);
                     }
-                    const T sharp = gamma * ph * (T(1) - ph);
+                    T sharp = gamma * ph * (T(1) - ph);
+                    // LEARNED flux closure (no reference counterpart): J = -D_t grad(phi)
+                    // with D_t = max(flux_coeff tau_t / 3, -0.1) adds 4 J to the first
+                    // moments that antidiff() carries as sharp * n
+                    if (p.closure == SSMD_CLOSURE_LEARNED && p.flux_coeff != 0.0)
+                        sharp -= T(4) * std::max(flux3 * fld[TAU_T][c], T(-0.1)) * ind;
                     for (int k = 0; k < 7; ++k) {
                         const T geq = L.geq(k, ph, ux, uy, uz);
                         const T hi = L.antidiff(k, sharp, normx, normy, normz);

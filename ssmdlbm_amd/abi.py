@@ -16,7 +16,7 @@ CLOSURE_NONE, CLOSURE_SMAGORINSKY, CLOSURE_LEARNED = 0, 1, 2
 MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 8, 64, 8961
 MLP_GRAD_NOALLOC_MAX = 65536
 OK, EINVAL, EUNSUPPORTED, ENOTREADY = 0, -1, -2, -3
-ABI_VERSION = 5
+ABI_VERSION = 6
 STEP_MACROS = 1
 REDUCE_DOUBLES = 8192
 POP_BYTES = {POP_F32: 4, POP_F16: 2, POP_BF16: 2}
@@ -63,6 +63,7 @@ class Params(C.Structure):
         ("grad_scale", C.c_double),
         ("tau_scale", C.c_double),
         ("tau_floor", C.c_double),
+        ("flux_coeff", C.c_double),
     ]
 
     @property

@@ -156,7 +156,7 @@ class Filter:
         torch.cuda.synchronize(sim.device)
 
     def plane(self, k: int) -> torch.Tensor:
-        """Filtered coarse plane k as (nz, ny, nx) (0 rho, 1-3 u, 4 phi, 5-10 S, 11-16 T)."""
+        """Filtered coarse plane k as (nz, ny, nx) (0 rho, 1-3 u, 4 phi, 5-10 S, 11-16 T, 17-19 J)."""
         return self.work[k * self.nc:(k + 1) * self.nc].view(self.coarse.shape)
 
 

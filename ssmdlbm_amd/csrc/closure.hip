@@ -566,7 +566,7 @@ __global__ void __launch_bounds__(256) k_pack(const float *__restrict__ p, uint8
 // twice the fine one.
 namespace sgs {
 
-enum Plane { kRho, kUx, kUy, kUz, kPhi, kS = 5, kT = 11, kPlanes = 17 };  // S, T: xx yy zz xy xz yz
+enum Plane { kRho, kUx, kUy, kUz, kPhi, kS = 5, kT = 11, kJ = 17, kPlanes = 20 };  // S, T: xx yy zz xy xz yz; J: x y z
 
 struct Fine {
     int nx, ny, nz;
@@ -582,8 +582,10 @@ __device__ __forceinline__ bool inner(int x, int y, int z, int nx, int ny, int n
 
 // Per coarse cell: Favre-filtered velocity, filtered rho, phi and strain rate
 // (fine S = -3 Pi / (2 rho tau), tau = 1/2 + 3 nu(phi)), and the subgrid
-// stress T = <rho u u> - <rho> u~ u~, all in coarse lattice units (velocity
-// x rv, strain rate x rt, stress x rv^2).  Ghost cells: rho 1, the rest 0.
+// stress T = <rho u u> - <rho> u~ u~ and the subgrid flux of the phase field
+// J = <phi u> - <phi> <u> (plain box averages), all in coarse lattice units
+// (velocity and flux x rv, strain rate x rt, stress x rv^2).  Ghost cells:
+// rho 1, the rest 0.
 __global__ void __launch_bounds__(256) k_filter(Fine f, int cnx, int cny, int cnz, float *__restrict__ out) {
     const size_t nc = size_t(cnx) * cny * cnz;
     const size_t c = size_t(blockIdx.x) * 256u + threadIdx.x;
@@ -596,6 +598,7 @@ __global__ void __launch_bounds__(256) k_filter(Fine f, int cnx, int cny, int cn
         v[kRho] = 1.f;
     } else {
         float r = 0, mu[3] = {0, 0, 0}, muu[6] = {0, 0, 0, 0, 0, 0}, ph = 0, st[6] = {0, 0, 0, 0, 0, 0};
+        float us[3] = {0, 0, 0}, pu[3] = {0, 0, 0};  // sum u, sum phi u
         for (int dz = 0; dz < 2; ++dz)
             for (int dy = 0; dy < 2; ++dy)
                 for (int dx = 0; dx < 2; ++dx) {
@@ -608,7 +611,11 @@ __global__ void __launch_bounds__(256) k_filter(Fine f, int cnx, int cny, int cn
                     r += rho;
                     ph += p;
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) mu[a] += rho * u[a];
+                    for (int a = 0; a < 3; ++a) {
+                        mu[a] += rho * u[a];
+                        us[a] += u[a];
+                        pu[a] += p * u[a];
+                    }
                     muu[0] += rho * u[0] * u[0];
                     muu[1] += rho * u[1] * u[1];
                     muu[2] += rho * u[2] * u[2];
@@ -624,6 +631,8 @@ __global__ void __launch_bounds__(256) k_filter(Fine f, int cnx, int cny, int cn
         v[kUy] = f.rv * ut[1];
         v[kUz] = f.rv * ut[2];
         v[kPhi] = ph * 0.125f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[kJ + a] = f.rv * (pu[a] * 0.125f - (ph * 0.125f) * (us[a] * 0.125f));
         const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
         const float rv2 = f.rv * f.rv;
 #pragma unroll

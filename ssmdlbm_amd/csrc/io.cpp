@@ -74,7 +74,7 @@ bool same_case(const ssmd_params &a, const ssmd_params &b) {
            a.visc_oil == b.visc_oil && a.sigma == b.sigma && a.gamma == b.gamma && a.perturb == b.perturb &&
            a.seed == b.seed && same_jet(a.water, b.water) && same_jet(a.oil, b.oil) && a.closure == b.closure &&
            a.cs == b.cs && a.strain_scale == b.strain_scale && a.grad_scale == b.grad_scale &&
-           a.tau_scale == b.tau_scale && a.tau_floor == b.tau_floor;
+           a.tau_scale == b.tau_scale && a.tau_floor == b.tau_floor && a.flux_coeff == b.flux_coeff;
 }
 
 // Nominal jet diameter: the stated one, else the even diameter of the disc.

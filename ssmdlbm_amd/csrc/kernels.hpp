@@ -49,6 +49,7 @@ struct KParams {
     int closure;                 // SSMD_CLOSURE_*
     float smag;                  // 18 Cs^2
     float strain_scale, grad_scale;
+    float flux3;                 // LEARNED phase-field flux closure: flux_coeff / 3
 };
 
 struct Macro {
@@ -321,7 +322,8 @@ __device__ __forceinline__ uint32_t bf16_pair(float a, float b) {
 }
 template <class MP>
 __device__ __forceinline__ float closure_tau(const KParams &k, MP macp, const Cell &m, uint32_t c, float ph,
-                                             float grad, float tau0, bool write_inputs) {
+                                             float grad, float tau0, bool write_inputs, float &tau_t) {
+    tau_t = 0.f;
     if (k.closure == SSMD_CLOSURE_SMAGORINSKY) {
         const float q = m.pxx * m.pxx + m.pyy * m.pyy + m.pzz * m.pzz +
                         2.0f * (m.pxy * m.pxy + m.pxz * m.pxz + m.pyz * m.pyz);
@@ -329,7 +331,8 @@ __device__ __forceinline__ float closure_tau(const KParams &k, MP macp, const Ce
                                                                          __builtin_amdgcn_rcpf(m.rho)));
     }
     const auto mac = macro_at(macp);
-    const float tau = tau0 + global_ptr(mac->tau_t)[c];
+    tau_t = global_ptr(mac->tau_t)[c];
+    const float tau = tau0 + tau_t;
     const float s = -1.5f * k.strain_scale * __builtin_amdgcn_rcpf(m.rho * tau);
     uint4 w;
     w.x = bf16_pair(m.pxx * s, m.pyy * s);
@@ -394,10 +397,11 @@ __device__ __forceinline__ void surface_force(const float (&s_n)[4][4][NP], int 
 template <int Q>
 __device__ __forceinline__ void stored_phi_normal(const KParams &k, const float *__restrict__ phi, int x, int y,
                                                   int z, bool active, const float4 &nc, float &nx, float &ny,
-                                                  float &nz) {
+                                                  float &nz, float &gs) {
     nx = nc.x;
     ny = nc.y;
     nz = nc.z;
+    gs = nc.w;
     const bool fx_lo = x == 1, fx_hi = x == k.nx - 2, fy_lo = y == 1, fy_hi = y == k.ny - 2;
     const bool fz_lo = z == 1, fz_hi = z == k.nz - 2;
     if (!(active && (fx_lo || fx_hi || fy_lo || fy_hi || fz_lo || fz_hi))) return;
@@ -439,6 +443,7 @@ __device__ __forceinline__ void stored_phi_normal(const KParams &k, const float 
     nx = gx * inv;
     ny = gy * inv;
     nz = gz * inv;
+    gs = ind;
 }
 
 // Bytes of global memory, for pointers the compiler must not lose the
@@ -695,9 +700,9 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
 
         // (4) normal of the stored phi for the phase update (interior lanes
         //     only: a ghost-column lane's x -+ 1 leaves the row)
-        float nx, ny, nz;
+        float nx, ny, nz, gs;
         const float ph = s_m[z % 5][(ty + 2) * HX + tx + 2];
-        stored_phi_normal<Q>(k, phi, x, y, z, active, nc, nx, ny, nz);
+        stored_phi_normal<Q>(k, phi, x, y, z, active, nc, nx, ny, nz, gs);
         const bool fy_lo = y == 1, fy_hi = y == k.ny - 2, fz_lo = z == 1, fz_hi = z == k.nz - 2;
 
         // (5) moments; then the pull of plane z+D; then the collision.  A
@@ -707,9 +712,10 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         const Cell m = rest_or(active, moments<Q, P>(fr, kx, ky, kz));
         if (z + D < ze) pull(z + D, fr);
         const float nu = __builtin_fmaf(ph, k.visc_diff, k.visc_water);
+        float tau_t = 0.f;
         const float omco = 1.0f - fast_rcp(k.closure == SSMD_CLOSURE_NONE
                                                ? 0.5f + 3.0f * nu
-                                               : closure_tau(k, macp, m, c, ph, nc.w, 0.5f + 3.0f * nu, active));
+                                               : closure_tau(k, macp, m, c, ph, nc.w, 0.5f + 3.0f * nu, active, tau_t));
         // uniform plane base (the previous one's plus N, as in the pull) +
         // shared 32-bit byte offset for every store; non-temporal: the next
         // step reads these from HBM, nothing here does
@@ -717,7 +723,11 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         GChar *splane = reinterpret_cast<GChar *>(reinterpret_cast<uintptr_t>(fout));
         const size_t SB = size_t(N0) * sizeof(PT);
         store_pops<P>(so, splane, SB, [&](auto &&emit) { collide<Q>(m, omco, kx, ky, kz, emit); });
-        const float sharp = k.gamma * ph * (1.0f - ph);
+        float sharp = k.gamma * ph * (1.0f - ph);
+        // LEARNED flux closure: J = -D_t grad(phi), D_t = max(flux_coeff tau_t / 3, -0.1),
+        // enters the emitted first moments 4 J as -4 D_t |grad phi| along the normal
+        if (k.closure == SSMD_CLOSURE_LEARNED && k.flux3 != 0.f && active)
+            sharp -= 4.0f * fmaxf(k.flux3 * tau_t, -0.1f) * gs;
         const uint32_t go = c * 4u;
         GChar *gplane = reinterpret_cast<GChar *>(reinterpret_cast<uintptr_t>(gout));
         const size_t GB = size_t(N0) * 4u;

@@ -75,6 +75,7 @@ KParams make_kparams(const ssmd_params &p) {
     k.smag = float(18.0 * p.cs * p.cs);
     k.strain_scale = float(p.strain_scale);
     k.grad_scale = float(p.grad_scale);
+    k.flux3 = float(p.flux_coeff / 3.0);
     return k;
 }
 
@@ -371,6 +372,8 @@ int ssmd_validate(const ssmd_params *p) {
         return SSMD_EINVAL;
     // tau = 1/2 + 3 nu(phi) + tau_t must stay above 1/2 with margin
     if (!(p->tau_floor <= 0.0 && p->tau_floor >= -0.9 * 3.0 * std::min(p->visc_water, p->visc_oil)))
+        return SSMD_EINVAL;
+    if (!std::isfinite(p->flux_coeff) || (p->flux_coeff != 0.0 && p->closure != SSMD_CLOSURE_LEARNED))
         return SSMD_EINVAL;
     return SSMD_OK;
 }

@@ -42,6 +42,7 @@ int main(void) {
   printf("%zu %zu %zu %zu\\n", offsetof(ssmd_params, closure), offsetof(ssmd_params, cs),
          offsetof(ssmd_params, tau_floor), offsetof(ssmd_fields, mlp));
   printf("%d %d %d\\n", SSMD_ABI_VERSION, SSMD_MLP_PARAMS, SSMD_CLOSURE_LEARNED);
+  printf("%zu\\n", offsetof(ssmd_params, flux_coeff));
   return 0;
 }
 """)
@@ -56,6 +57,7 @@ int main(void) {
     assert out[3] == [abi.Params.closure.offset, abi.Params.cs.offset, abi.Params.tau_floor.offset,
                       abi.Fields.mlp.offset]
     assert out[4] == [abi.ABI_VERSION, abi.MLP_PARAMS, abi.CLOSURE_LEARNED]
+    assert out[5] == [abi.Params.flux_coeff.offset]
 
 
 def test_reference_default_constants():
@@ -119,6 +121,16 @@ def test_validation_rejects_bad_parameters():
         b = p.copy()
         b.tau_floor = floor
         assert lib.ssmd_validate(b) == rc, floor
+    # the phase-field flux closure rides on the LEARNED closure's tau_t
+    for closure, coeff, rc in ((abi.CLOSURE_NONE, 0.0, 0), (abi.CLOSURE_NONE, 1.0, -1),
+                               (abi.CLOSURE_SMAGORINSKY, 1.0, -1), (abi.CLOSURE_LEARNED, 1.0, 0),
+                               (abi.CLOSURE_LEARNED, -3.0, 0), (abi.CLOSURE_LEARNED, float("inf"), -1),
+                               (abi.CLOSURE_LEARNED, float("nan"), -1)):
+        b = p.copy()
+        b.closure, b.flux_coeff = closure, coeff
+        if closure == abi.CLOSURE_SMAGORINSKY:
+            b.cs = 0.1
+        assert lib.ssmd_validate(b) == rc, (closure, coeff)
 
 
 def test_tile_shapes():

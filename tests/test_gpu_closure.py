@@ -240,6 +240,47 @@ def test_learned_closure_inputs_and_update(gpu, q):
     assert np.abs(got - want).max() <= 1e-2 * rms + 1e-9
 
 
+@pytest.mark.parametrize("q", [19, 27])
+def test_learned_flux_closure_matches_oracle(gpu, q):
+    """The phase-field flux closure (flux_coeff = 30: D_t = max(10 tau_t,
+    -0.1)) on a LEARNED jet with a shared random tau_t in [-0.015, 0.03],
+    so about a tenth of the cells sit at the -0.1 clamp: ten steps without
+    the network pass against the FP64 oracle (1e-5 relative L2 on every
+    moment and phi).  The same steps with flux_coeff = 0 move phi by ~5%:
+    the closure is exercised, not a no-op."""
+    p = learned_case(q)
+    p.flux_coeff = 30.0
+    sim = Simulation(p)
+    sim.init()
+    o = Oracle(p, fp64=True, threads=cpu_threads())
+    o.init()
+    sim.step(20)
+    o.step(20)
+    sim.synchronize()
+    tau_t = np.random.default_rng(23).uniform(-0.015, 0.03, p.cells).astype(np.float32)
+    sim.t["tau_t"].copy_(torch.tensor(tau_t))
+    o.set_field("tau_t", tau_t.astype(np.float64))
+    torch.cuda.synchronize()
+    for s in range(10):
+        last = s == 9
+        sim.launch_phase()
+        sim.launch_lattice(macros=last)
+        sim.launch_boundaries(macros=last)
+        sim.swap()
+    o.step(10)
+    sim.synchronize()
+    errs = {k: rel_l2(sim.field(k), o.field(k)) for k in MACRO}
+    assert max(errs.values()) < 1e-5, errs
+    p0 = p.copy()
+    p0.flux_coeff = 0.0
+    o0 = Oracle(p0, fp64=True, threads=cpu_threads())
+    o0.init()
+    o0.step(20)
+    o0.set_field("tau_t", tau_t.astype(np.float64))
+    o0.step(10)
+    assert rel_l2(o0.field("phi"), o.field("phi")) > 1e-2
+
+
 def test_learned_closure_graph_replay_matches_eager(gpu):
     p = learned_case()
     net = Network(random_params(31, bias=0.2), scales=Scales(200.0, 4.0, 0.02))
@@ -263,12 +304,16 @@ def test_learned_closure_graph_replay_matches_eager(gpu):
 @pytest.mark.parametrize("scaling", ["acoustic", "diffusive"])
 def test_sgs_samples_analytic(gpu, scaling):
     """Linear fields on the fine grid give closed forms after the 2x box filter:
-    u_x = a y, phi = b x, Pi = diag(p0, -p0, 0), rho = 1; in coarse lattice
-    units, with rt = dt_c / dt_f = 4 nu_c / nu_f (2 acoustic, 4 diffusive) and
-    rv = rt / 2:
-      u~_x = rv a (2 cy - 1/2), phi~ = b (2 cx - 1/2), S = rt (-3 Pi / (2 tau_f)),
-      T_xx = rv^2 a^2 / 4 (variance of y over a block), other T = 0,
-      nu_t = -(dev T : S) / (2 S:S) = rv^2 a^2 / (16 s) with s = 1.5 rt p0 / tau_f."""
+    u_x = a y, u_z = c x, phi = b x, Pi = diag(p0, -p0, 0), rho = 1; in coarse
+    lattice units, with rt = dt_c / dt_f = 4 nu_c / nu_f (2 acoustic, 4
+    diffusive) and rv = rt / 2:
+      u~_x = rv a (2 cy - 1/2), u~_z = rv c (2 cx - 1/2), phi~ = b (2 cx - 1/2),
+      S = rt (-3 Pi / (2 tau_f)),
+      T_xx = rv^2 a^2 / 4 (variance of y over a block), T_zz = rv^2 c^2 / 4,
+      other T = 0 (x and y vary independently over a block),
+      J_z = <phi u_z> - <phi> <u_z> = rv b c / 4 (covariance of x and x), J_x = J_y = 0,
+      nu_t = -(dev T : S) / (2 S:S) = rv^2 a^2 / (16 s) with s = 1.5 rt p0 / tau_f
+      (T_zz meets S_zz = 0)."""
     from ssmdlbm_amd.closure_study import Filter
     coarse = cases.JET_D3Q19_SMALL.with_(nx=12, ny=10, nz=14, diameter=4).params()
     coarse.oil.enabled = 0  # only the geometry matters here; no inlet near the periodic faces
@@ -280,13 +325,13 @@ def test_sgs_samples_analytic(gpu, scaling):
     rv = rt / 2
     fine.visc_water = fine.visc_oil = nu
     tau_f = 0.5 + 3 * nu
-    a, b, p0 = 0.01, 0.02, 1e-4
+    a, b, c, p0 = 0.01, 0.02, 0.03, 1e-4
     sim = Simulation(fine)
     nz, ny, nx = fine.shape
     zz, yy, xx = torch.meshgrid(torch.arange(nz), torch.arange(ny), torch.arange(nx), indexing="ij")
     dev = sim.t["rho"].device
     vals = {"rho": torch.ones(fine.shape), "ux": a * yy.float(), "uy": torch.zeros(fine.shape),
-            "uz": torch.zeros(fine.shape), "phi": b * xx.float(), "pxx": torch.full(fine.shape, p0),
+            "uz": c * xx.float(), "phi": b * xx.float(), "pxx": torch.full(fine.shape, p0),
             "pyy": torch.full(fine.shape, -p0)}
     for k in ("pzz", "pxy", "pxz", "pyz"):
         vals[k] = torch.zeros(fine.shape)
@@ -303,12 +348,20 @@ def test_sgs_samples_analytic(gpu, scaling):
     pl = lambda k: flt.plane(k).cpu().double().numpy()
     np.testing.assert_allclose(pl(0)[inner], 1.0, rtol=1e-6)
     np.testing.assert_allclose(pl(1)[inner], (rv * a * (2 * Y - 0.5))[inner], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(pl(3)[inner], (rv * c * (2 * X - 0.5))[inner], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(pl(4)[inner], (b * (2 * X - 0.5))[inner], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(pl(5)[inner], -s, rtol=1e-5)
     np.testing.assert_allclose(pl(6)[inner], s, rtol=1e-5)
     np.testing.assert_allclose(pl(11)[inner], rv * rv * a * a / 4, rtol=2e-3)  # nearby squares differ in FP32
-    for k in (7, 8, 9, 10, 12, 13, 14, 15, 16):
+    np.testing.assert_allclose(pl(13)[inner], rv * rv * c * c / 4, rtol=2e-3)
+    np.testing.assert_allclose(pl(19)[inner], rv * b * c / 4, rtol=2e-3)
+    for k in (7, 8, 9, 10, 12, 14):
         assert np.abs(pl(k)[inner]).max() < 1e-5 * rv * rv * a * a, k
+    # T_xz, T_yz: FP32 differences of block means of u_x u_z ~ a c x y
+    for k in (15, 16):
+        assert np.abs(pl(k)[inner]).max() < 1e-6 * rv * rv * a * c * fine.nx * fine.ny, k
+    for k in (17, 18):
+        assert np.abs(pl(k)[inner]).max() < 1e-4 * rv * b * c, k
     target = flt.target.cpu().double().numpy().reshape(coarse.shape)
     weight = flt.weight.cpu().double().numpy().reshape(coarse.shape)
     np.testing.assert_allclose(target[inner], 3 * rv * rv * a * a / (16 * s) / 0.01, rtol=3e-3)

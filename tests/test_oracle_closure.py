@@ -180,3 +180,70 @@ def test_calibrate_command_line_accepts_the_closure_parameters(tmp_path, capsys)
     assert "runs on the GPU" in capsys.readouterr().err
     with pytest.raises(SystemExit):
         closure_study.main(["calibrate", "--out", str(tmp_path), "--report", "r.json", "--capillary-scale", "2"])
+
+
+# -- phase-field flux closure --------------------------------------------------
+def _flux_box(q, coeff):
+    """Periodic 32 x 4 x 4 interior, no flow, no surface tension, no
+    sharpening: only the D3Q7 diffusion and the closure flux act on phi."""
+    from ssmdlbm_amd import abi, cases
+    p = cases.closed_box(6, q=q, sigma=0.0, gamma=0.0)
+    p.nx = 34
+    p.closure = abi.CLOSURE_LEARNED
+    p.tau_scale = 1.0
+    p.flux_coeff = coeff
+    return p
+
+
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("coeff,tau_t", [(0.0, 0.2), (1.5, 0.1), (2.0, -0.12), (30.0, -0.03)])
+def test_flux_closure_is_gradient_diffusion(q, coeff, tau_t):
+    """With u = 0 and gamma = 0 the D3Q7 update (populations w_k (phi +
+    c_k . M) leaving each cell, w_0 = 1/4, w_k = 1/8) with M = -4 D grad phi
+    and the isotropic gradient (phi(x+1) - phi(x-1)) / 2 maps the periodic
+    mode sin(k x) to lambda sin(k x), lambda = 1 - (1 - cos k) / 4 - D sin^2 k:
+    diffusion with mobility 1/8 + D for long waves.  The closure's
+    D = max(flux_coeff tau_t / 3, -0.1) is read off the decay over 200
+    steps (tau_t held uniform; the last case is clamped at -0.1).  The ghost
+    layer's phi is refreshed after each phase update so the stencil sees the
+    periodic image of the current field (the step itself refreshes it one
+    kernel later, after the collision)."""
+    p = _flux_box(q, coeff)
+    o = O.Oracle(p, fp64=True, threads=2)
+    o.init()
+    nz, ny, nx = p.shape
+    L = nx - 2
+    k = 2.0 * np.pi / L
+    xi = np.arange(nx) - 1.0  # interior x' = 0..L-1; ghosts at -1 and L are periodic images
+    mode = np.sin(k * xi)[None, None, :] * np.ones(p.shape)
+    phi0, amp0 = 0.4, 0.1
+    phi = phi0 + amp0 * mode
+    w7 = np.array([O.load().oracle_w(7, i) for i in range(7)])
+    o.set_pops("g", w7[:, None, None, None] * phi[None])  # arrivals that sum to phi at every cell
+    o.set_field("tau_t", np.full(p.shape, tau_t))
+
+    def wrap_ghosts(f):
+        f[:, :, 0], f[:, :, -1] = f[:, :, -2], f[:, :, 1]
+        f[:, 0, :], f[:, -1, :] = f[:, -2, :], f[:, 1, :]
+        f[0], f[-1] = f[-2], f[1]
+        return f
+
+    steps = 200
+    for s in range(steps):
+        o.k_phase()
+        o.set_field("phi", wrap_ghosts(o.field("phi")))
+        o.k_forces()
+        o.k_moments()
+        o.k_collide()
+        o.k_boundaries(s)
+    o.k_phase()
+    got = o.field("phi")[1:-1, 1:-1, 1:-1]
+    sines = np.sin(k * xi[1:-1])
+    amp = np.mean((got - phi0) * sines[None, None, :]) / np.mean(sines ** 2)
+    D = max(coeff * tau_t / 3.0, -0.1)
+    lam = 1.0 - (1.0 - np.cos(k)) / 4.0 - D * np.sin(k) ** 2
+    # the flux enters through the normal grad/(|grad| + 1e-9): relative 1e-9 / |grad| ~ 1e-7 of it here
+    assert amp / amp0 == pytest.approx(lam ** steps, rel=1e-6)
+    # the mode stays a pure mode, mass is conserved
+    np.testing.assert_allclose(got, phi0 + amp * sines[None, None, :] * np.ones_like(got), rtol=0, atol=1e-7)
+    assert got.mean() == pytest.approx(phi0, rel=1e-13)
