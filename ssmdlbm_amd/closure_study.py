@@ -31,6 +31,7 @@ Command line (one GPU):
   python -m ssmdlbm_amd.closure_study noise --out DIR   seed-to-seed spread
   python -m ssmdlbm_amd.closure_study sensitivity --report DIR/eval_report.json --out DIR
   python -m ssmdlbm_amd.closure_study calibrate --report DIR/eval_report.json --out DIR
+  python -m ssmdlbm_amd.closure_study flux --report DIR/eval_report.json --out DIR
 """
 from __future__ import annotations
 
@@ -542,7 +543,9 @@ def calibrate_main(args) -> int:
     (sigma_coarse = s sigma), with --sharpening a parameter h its D3Q7
     sharpening coefficient (gamma_coarse = h gamma): coarse-grid closures of
     the capillary force and of the phase-field flux, the levers the interface
-    statistics answer to and tau_t cannot reach."""
+    statistics answer to and tau_t cannot reach.  With --flux a parameter f
+    is the flux_coeff of the interface-localised tau_t (D_t = f tau_t / 3): a
+    phase-field flux closure local to the interface."""
     from scipy.optimize import minimize
 
     out = Path(args.out)
@@ -557,39 +560,43 @@ def calibrate_main(args) -> int:
     fine_train = run_fine(g, train, steps, every, stat_from, [], z_range, kmax)
     history = []
 
-    def errors(jet, fine, kappa, beta, s=1.0, h=1.0):
+    def errors(jet, fine, kappa, beta, s=1.0, h=1.0, f=0.0):
         net = Network(interface_network(kappa, beta),
                       scales=Scales(strain_scale=1.0, grad_scale=args.grad_scale, tau_scale=1e-3, tau_floor=floor))
 
         def edit(p):
             p.sigma = p.sigma * s
             p.gamma = p.gamma * h
+            p.flux_coeff = f
 
         r = run_coarse(g, jet, abi.CLOSURE_LEARNED, net, steps, every, stat_from, z_range, kmax, tune=False,
-                       edit=edit if (s, h) != (1.0, 1.0) else None)
+                       edit=edit if (s, h, f) != (1.0, 1.0, 0.0) else None)
         e = compare(r, fine, z_range, kmax) if r["finite"] else None
         return e, r
 
-    # parameter vector: kappa, beta, then s (--capillary), then h (--sharpening)
-    names = ["kappa", "beta"] + (["s"] if args.capillary else []) + (["h"] if args.sharpening else [])
+    # parameter vector: kappa, beta, then s (--capillary), h (--sharpening), f (--flux)
+    names = (["kappa", "beta"] + (["s"] if args.capillary else []) + (["h"] if args.sharpening else []) +
+             (["f"] if args.flux else []))
 
     def unpack(v):
         d = dict(zip(names, (float(a) for a in v)))
-        return d["kappa"], d["beta"], d.get("s", 1.0), d.get("h", 1.0)
+        return d["kappa"], d["beta"], d.get("s", 1.0), d.get("h", 1.0), d.get("f", 0.0)
 
     def objective(v):
-        kappa, beta, s, h = unpack(v)
-        if not (0.5 <= s <= 2.0 and 0.25 <= h <= 2.0):  # keep to stable, studied ranges
+        kappa, beta, s, h, f = unpack(v)
+        if not (0.5 <= s <= 2.0 and 0.25 <= h <= 2.0 and abs(f) <= 100.0):  # keep to stable, studied ranges
             return 10.0
-        e, r = errors(train, fine_train, kappa, beta, s, h)
+        e, r = errors(train, fine_train, kappa, beta, s, h, f)
         j = max(e.values()) if e else 10.0
-        history.append({"kappa": kappa, "beta": beta, "sigma_scale": s, "gamma_scale": h, "errors": e,
-                        "objective": j, "tau_t_mean": r.get("tau_t_mean"), "tau_t_max": r.get("tau_t_max")})
-        log(f"kappa {kappa:8.3f} beta {beta:8.3f} s {s:6.3f} h {h:6.3f}: max error {j:.4f} {e}")
+        history.append({"kappa": kappa, "beta": beta, "sigma_scale": s, "gamma_scale": h, "flux_coeff": f,
+                        "errors": e, "objective": j, "tau_t_mean": r.get("tau_t_mean"),
+                        "tau_t_max": r.get("tau_t_max")})
+        log(f"kappa {kappa:8.3f} beta {beta:8.3f} s {s:6.3f} h {h:6.3f} f {f:7.3f}: max error {j:.4f} {e}")
         return j
 
-    x0 = [args.kappa0, args.beta0] + ([1.0] if args.capillary else []) + ([1.0] if args.sharpening else [])
-    steps0 = {"kappa": args.step, "beta": args.step, "s": args.sigma_step, "h": args.gamma_step}
+    x0 = ([args.kappa0, args.beta0] + ([1.0] if args.capillary else []) + ([args.h0] if args.sharpening else []) +
+          ([args.flux0] if args.flux else []))
+    steps0 = {"kappa": args.step, "beta": args.step, "s": args.sigma_step, "h": args.gamma_step, "f": args.flux_step}
     simplex = [list(x0)]
     for i, nm in enumerate(names):
         v = list(x0)
@@ -597,25 +604,141 @@ def calibrate_main(args) -> int:
         simplex.append(v)
     res = minimize(objective, x0=np.array(x0), method="Nelder-Mead",
                    options={"maxfev": args.evals, "initial_simplex": np.array(simplex)})
-    kappa, beta, s_best, h_best = unpack(res.x)
+    kappa, beta, s_best, h_best, f_best = unpack(res.x)
     held_fine = ref["filtered_fine"]
-    e_best, _ = errors(held, held_fine, kappa, beta, s_best, h_best)
+    e_best, _ = errors(held, held_fine, kappa, beta, s_best, h_best, f_best)
     e_none = ref["errors"]["none"]
     rep = {"train_jet": asdict(train), "held_out_jet": asdict(held), "floor": floor, "grad_scale": args.grad_scale,
            "history": history,
            "best": {"kappa": kappa, "beta": beta, "sigma_scale": s_best, "gamma_scale": h_best,
-                    "train_objective": float(res.fun)},
+                    "flux_coeff": f_best, "train_objective": float(res.fun)},
            "held_out": {"calibrated": e_best, "none": e_none}}
     (out / "calibrate_report.json").write_text(json.dumps(rep, indent=1))
-    log(f"best kappa {kappa:.3f} beta {beta:.3f} sigma scale {s_best:.3f} gamma scale {h_best:.3f}; "
+    log(f"best kappa {kappa:.3f} beta {beta:.3f} sigma scale {s_best:.3f} gamma scale {h_best:.3f} "
+        f"flux_coeff {f_best:.3f}; "
         f"held-out calibrated {e_best}; none {e_none}")
+    return 0
+
+
+# -- phase-field flux closure -------------------------------------------------
+class FluxFit:
+    """A-priori least squares of the subgrid flux of the phase field, J =
+    <phi u> - <phi><u> (filter planes 17-19), against gradient diffusion
+    -D grad(phi-bar) (central differences on the coarse grid, over cells two
+    or more layers inside): D = flux_coeff * nu_t per network (nu_t = tau_t / 3
+    on the filter's inputs; the flux_coeff closure) and D = const.  Sums in
+    FP64 over every snapshot added."""
+
+    def __init__(self, nets: dict[str, Network]):
+        self.nets = nets
+        self.s = {k: 0.0 for k in ("jj", "jg", "gg")}
+        for n in nets:
+            self.s[f"jg_{n}"] = 0.0
+            self.s[f"gg_{n}"] = 0.0
+        self.snapshots = 0
+
+    def add(self, flts: dict[str, Filter]) -> None:
+        flt = next(iter(flts.values()))
+        phi = flt.plane(4).double()
+        core = (slice(2, -2),) * 3
+        gx = 0.5 * (phi[2:-2, 2:-2, 3:-1] - phi[2:-2, 2:-2, 1:-3])
+        gy = 0.5 * (phi[2:-2, 3:-1, 2:-2] - phi[2:-2, 1:-3, 2:-2])
+        gz = 0.5 * (phi[3:-1, 2:-2, 2:-2] - phi[1:-3, 2:-2, 2:-2])
+        jx, jy, jz = (flt.plane(17 + a).double()[core] for a in range(3))
+        mjg = -(jx * gx + jy * gy + jz * gz)  # -J . grad phi
+        gg = gx * gx + gy * gy + gz * gz
+        self.s["jj"] += float((jx * jx + jy * jy + jz * jz).sum())
+        self.s["jg"] += float(mjg.sum())
+        self.s["gg"] += float(gg.sum())
+        for name, net in self.nets.items():
+            f = flts[name]
+            y = net.forward(f.feat.view(-1, 8)).double().view(f.coarse.shape)
+            nu = torch.clamp(net.scales.tau_scale * y, min=net.scales.tau_floor)[core] / 3.0
+            self.s[f"jg_{name}"] += float((nu * mjg).sum())
+            self.s[f"gg_{name}"] += float((nu * nu * gg).sum())
+        self.snapshots += 1
+
+    def result(self) -> dict:
+        s = self.s
+        out = {"snapshots": self.snapshots, "flux_sq_sum": s["jj"],
+               "constant": {"D": s["jg"] / s["gg"], "explained": s["jg"] ** 2 / (s["gg"] * s["jj"])}}
+        for n in self.nets:
+            a, b = s[f"jg_{n}"], s[f"gg_{n}"]
+            out[n] = {"flux_coeff": a / b if b > 0 else 0.0, "explained": a * a / (b * s["jj"]) if b > 0 else 0.0}
+        return out
+
+
+def flux_main(args) -> int:
+    """The phase-field flux closure: flux_coeff fitted a priori (FluxFit) on
+    the filtered fine runs of `--runs` training jets, for the trained network
+    and for the calibrated interface network; then the held-out jet of an
+    evaluation report run coarse with each network at flux_coeff 0 and at the
+    fitted value, scored against that report's filtered fine statistics."""
+    out = Path(args.out)
+    out.mkdir(parents=True, exist_ok=True)
+    ref = json.loads(Path(args.report).read_text())
+    g = Geometry(**dict(ref["geometry"]))
+    held = Jet(**ref["jet"])
+    z_range, kmax, stat_from, steps, every = tuple(ref["z_range"]), ref["kmax"], ref["stat_from"], ref["steps"], ref["every"]
+    nets = {"trained": Network.load(args.net), "interface": Network.load(args.interface_net)}
+    rng = np.random.default_rng(args.seed)
+    jets = [Jet(float(rng.uniform(*args.u_range)), float(rng.uniform(*args.sigma_range)), int(args.seed * 100 + j))
+            for j in range(args.runs)]
+    log(f"flux fit on jets {[asdict(j) for j in jets]}")
+    fit = FluxFit(nets)
+    snaps = [int(v) for v in np.linspace(steps // 2, steps, args.snapshots)]
+    per_jet = []
+    for jet in jets:
+        fp = fine_params(g, jet)
+        flts = {n: Filter(fp, coarse_params(g, jet, abi.CLOSURE_LEARNED, net.scales)) for n, net in nets.items()}
+        one = FluxFit(nets)
+        r = g.fine_steps
+        sim = Simulation(fp)
+        sim.init()
+        sim.step(2)
+        sim.tune(reps=4)
+        t0 = time.perf_counter()
+        for m in snaps:
+            if r * m - 1 > sim.step_index:
+                sim.graph_run(r * m - 1 - sim.step_index)
+            sim.step(1, macros=True)
+            for f in flts.values():
+                f(sim)
+            fit.add(flts)
+            one.add(flts)
+            log(f"fine u={jet.u:.3f} sigma={jet.sigma:.3f}: snapshot at coarse step {m} "
+                f"({time.perf_counter() - t0:.0f} s)")
+        sim.synchronize()
+        if not torch.isfinite(sim.t["uz"]).all():
+            raise RuntimeError("fine run diverged")
+        sim.close()
+        del sim, flts
+        torch.cuda.empty_cache()
+        per_jet.append({"jet": asdict(jet), "fit": one.result()})
+        log(f"jet fit: {per_jet[-1]['fit']}")
+    apriori = fit.result()
+    log(f"a-priori fit over {len(jets)} jets: {apriori}")
+    post = {}
+    for name, net in nets.items():
+        for coeff in (0.0, apriori[name]["flux_coeff"]):
+            def edit(p, c=coeff):
+                p.flux_coeff = c
+            res = run_coarse(g, held, abi.CLOSURE_LEARNED, net, steps, every, stat_from, z_range, kmax, tune=False,
+                             edit=edit)
+            e = compare(res, ref["filtered_fine"], z_range, kmax) if res["finite"] else None
+            post[f"{name}_flux{coeff:.4g}"] = {"network": name, "flux_coeff": coeff, "errors": e,
+                                                "finite": res["finite"], "tau_t_mean": res.get("tau_t_mean")}
+            log(f"held out, {name} network, flux_coeff {coeff:.4g}: {e}")
+    rep = {"train_jets": [asdict(j) for j in jets], "snapshots": snaps, "apriori": apriori, "per_jet": per_jet,
+           "held_out_jet": asdict(held), "a_posteriori": post, "none": ref["errors"]["none"]}
+    (out / "flux_report.json").write_text(json.dumps(rep, indent=1))
     return 0
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
-    for name in ("train", "eval", "noise", "sensitivity", "calibrate"):
+    for name in ("train", "eval", "noise", "sensitivity", "calibrate", "flux"):
         s = sub.add_parser(name)
         s.add_argument("--out", required=True)
         s.add_argument("--nx", type=int, default=128)
@@ -666,12 +789,26 @@ def main(argv=None) -> int:
     ca.add_argument("--sharpening", action="store_true",
                     help="also fit h, the coarse D3Q7 sharpening scale (gamma_coarse = h gamma)")
     ca.add_argument("--gamma-step", type=float, default=-0.25, help="initial simplex step of h")
+    ca.add_argument("--h0", type=float, default=1.0, help="starting h (--sharpening)")
+    ca.add_argument("--flux", action="store_true",
+                    help="also fit f, the phase-field flux closure's flux_coeff (D_t = f tau_t / 3)")
+    ca.add_argument("--flux0", type=float, default=0.0, help="starting f (--flux)")
+    ca.add_argument("--flux-step", type=float, default=3.0, help="initial simplex step of f")
+    fx = sub.choices["flux"]
+    fx.add_argument("--report", required=True, help="the held-out jet's eval_report.json")
+    fx.add_argument("--net", default=str(Path(__file__).resolve().parent / "data" / "closure_jet_d3q27"))
+    fx.add_argument("--interface-net",
+                    default=str(Path(__file__).resolve().parent / "data" / "closure_jet_d3q27_interface"))
+    fx.add_argument("--runs", type=int, default=3)
+    fx.add_argument("--u-range", type=float, nargs=2, default=(0.03, 0.08))
+    fx.add_argument("--sigma-range", type=float, nargs=2, default=(0.01, 0.03))
+    fx.add_argument("--snapshots", type=int, default=6)
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("the closure study runs on the GPU", file=sys.stderr)
         return 1
     return {"train": train_main, "eval": eval_main, "noise": noise_main, "sensitivity": sensitivity_main,
-            "calibrate": calibrate_main}[args.cmd](args)
+            "calibrate": calibrate_main, "flux": flux_main}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -117,3 +117,36 @@ def test_compare_closed_forms():
     assert e["centreline_rel_l2"] == pytest.approx(0.1)
     assert e["spectrum_rel_l2"] == pytest.approx(0.1)  # bins 1..kmax only
     assert e["interface_area_rel"] == pytest.approx(0.05)
+
+
+class _Planes:
+    """A stand-in for the filter: planes 4 (phi) and 17-19 (J) as given."""
+
+    def __init__(self, phi, j):
+        self.p = {4: phi, 17: j[0], 18: j[1], 19: j[2]}
+
+    def plane(self, k):
+        return self.p[k]
+
+
+def test_flux_fit_recovers_gradient_diffusion():
+    """FluxFit's constant-D least squares: a subgrid flux that is exactly
+    -D grad(phi) (central differences) gives D back with all of |J|^2
+    explained; a flux orthogonal to grad(phi) gives D = 0."""
+    from ssmdlbm_amd.closure_study import FluxFit
+    rng = np.random.default_rng(5)
+    n = 12
+    phi = torch.from_numpy(rng.uniform(0, 1, (n, n, n)))
+    c = lambda a, ax: 0.5 * (torch.roll(a, -1, ax) - torch.roll(a, 1, ax))
+    g = [c(phi, 2), c(phi, 1), c(phi, 0)]
+    D = 0.037
+    fit = FluxFit({})
+    fit.add({"x": _Planes(phi, [-D * gi for gi in g])})
+    r = fit.result()
+    assert r["constant"]["D"] == pytest.approx(D, rel=1e-12)
+    assert r["constant"]["explained"] == pytest.approx(1.0, rel=1e-12)
+    # J = grad(phi) x e_z is orthogonal to grad(phi) everywhere
+    fit = FluxFit({})
+    fit.add({"x": _Planes(phi, [g[1], -g[0], torch.zeros_like(phi)])})
+    r = fit.result()
+    assert abs(r["constant"]["D"]) < 1e-12 and r["constant"]["explained"] < 1e-20
