@@ -626,12 +626,23 @@ class FluxFit:
     <phi u> - <phi><u> (filter planes 17-19), against gradient diffusion
     -D grad(phi-bar) (central differences on the coarse grid, over cells two
     or more layers inside): D = flux_coeff * nu_t per network (nu_t = tau_t / 3
-    on the filter's inputs; the flux_coeff closure) and D = const.  Sums in
-    FP64 over every snapshot added."""
+    on the filter's inputs; the flux_coeff closure) and D = const.  Two tensor
+    models are fitted beside them, J = C M with
+      gradient  M_a = sum_b d_b phi-bar d_b u-bar_a (the leading term of the box
+                filter's covariance for fields linear over a block: C = 1/16
+                for the 2x filter in coarse cells);
+      strain    M_a = sum_b S-bar_ab d_b phi-bar (the part of it a collision can
+                form from its own non-equilibrium moments).
+    Sums in FP64 over every snapshot added."""
+
+    MODELS = ("gradient", "strain")
 
     def __init__(self, nets: dict[str, Network]):
         self.nets = nets
         self.s = {k: 0.0 for k in ("jj", "jg", "gg")}
+        for m in self.MODELS:
+            self.s[f"jm_{m}"] = 0.0
+            self.s[f"mm_{m}"] = 0.0
         for n in nets:
             self.s[f"jg_{n}"] = 0.0
             self.s[f"gg_{n}"] = 0.0
@@ -650,6 +661,20 @@ class FluxFit:
         self.s["jj"] += float((jx * jx + jy * jy + jz * jz).sum())
         self.s["jg"] += float(mjg.sum())
         self.s["gg"] += float(gg.sum())
+
+        def grad(f):
+            return (0.5 * (f[2:-2, 2:-2, 3:-1] - f[2:-2, 2:-2, 1:-3]),
+                    0.5 * (f[2:-2, 3:-1, 2:-2] - f[2:-2, 1:-3, 2:-2]),
+                    0.5 * (f[3:-1, 2:-2, 2:-2] - f[1:-3, 2:-2, 2:-2]))
+        g = (gx, gy, gz)
+        du = [grad(flt.plane(1 + a).double()) for a in range(3)]  # du[a][b] = d_b u_a
+        grad_m = [sum(g[b] * du[a][b] for b in range(3)) for a in range(3)]
+        S = [flt.plane(5 + k).double()[core] for k in range(6)]  # xx yy zz xy xz yz
+        Sm = ((S[0], S[3], S[4]), (S[3], S[1], S[5]), (S[4], S[5], S[2]))
+        strain_m = [sum(Sm[a][b] * g[b] for b in range(3)) for a in range(3)]
+        for name, M in (("gradient", grad_m), ("strain", strain_m)):
+            self.s[f"jm_{name}"] += float((jx * M[0] + jy * M[1] + jz * M[2]).sum())
+            self.s[f"mm_{name}"] += float((M[0] * M[0] + M[1] * M[1] + M[2] * M[2]).sum())
         for name, net in self.nets.items():
             f = flts[name]
             y = net.forward(f.feat.view(-1, 8)).double().view(f.coarse.shape)
@@ -665,6 +690,10 @@ class FluxFit:
         for n in self.nets:
             a, b = s[f"jg_{n}"], s[f"gg_{n}"]
             out[n] = {"flux_coeff": a / b if b > 0 else 0.0, "explained": a * a / (b * s["jj"]) if b > 0 else 0.0}
+        for m in self.MODELS:
+            a, b = s[f"jm_{m}"], s[f"mm_{m}"]
+            if b > 0:
+                out[m] = {"C": a / b, "explained": a * a / (b * s["jj"])}
         return out
 
 

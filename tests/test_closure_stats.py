@@ -120,10 +120,16 @@ def test_compare_closed_forms():
 
 
 class _Planes:
-    """A stand-in for the filter: planes 4 (phi) and 17-19 (J) as given."""
+    """A stand-in for the filter: planes 1-3 (u), 4 (phi), 5-10 (S) and 17-19
+    (J) as given, zero where not given."""
 
-    def __init__(self, phi, j):
+    def __init__(self, phi, j, u=None, strain=None):
+        z = torch.zeros_like(phi)
         self.p = {4: phi, 17: j[0], 18: j[1], 19: j[2]}
+        for a in range(3):
+            self.p[1 + a] = z if u is None else u[a]
+        for k in range(6):
+            self.p[5 + k] = z if strain is None else strain[k]
 
     def plane(self, k):
         return self.p[k]
@@ -150,3 +156,26 @@ def test_flux_fit_recovers_gradient_diffusion():
     fit.add({"x": _Planes(phi, [g[1], -g[0], torch.zeros_like(phi)])})
     r = fit.result()
     assert abs(r["constant"]["D"]) < 1e-12 and r["constant"]["explained"] < 1e-20
+
+
+def test_flux_fit_tensor_models():
+    """A flux that is exactly C (grad phi . grad) u gives C back for the
+    gradient model with all of |J|^2 explained, and one that is exactly
+    C' S grad phi does the same for the strain model."""
+    from ssmdlbm_amd.closure_study import FluxFit
+    rng = np.random.default_rng(6)
+    n = 12
+    phi = torch.from_numpy(rng.uniform(0, 1, (n, n, n)))
+    u = [torch.from_numpy(rng.normal(0, 0.05, (n, n, n))) for _ in range(3)]
+    S = [torch.from_numpy(rng.normal(0, 1e-3, (n, n, n))) for _ in range(6)]
+    c = lambda a, ax: 0.5 * (torch.roll(a, -1, ax) - torch.roll(a, 1, ax))
+    g = [c(phi, 2), c(phi, 1), c(phi, 0)]
+    grad_model = [sum(g[b] * c(u[a], 2 - b) for b in range(3)) for a in range(3)]
+    Sm = ((S[0], S[3], S[4]), (S[3], S[1], S[5]), (S[4], S[5], S[2]))
+    strain_model = [sum(Sm[a][b] * g[b] for b in range(3)) for a in range(3)]
+    for name, M, C in (("gradient", grad_model, 0.0625), ("strain", strain_model, -0.4)):
+        fit = FluxFit({})
+        fit.add({"x": _Planes(phi, [C * m for m in M], u=u, strain=S)})
+        r = fit.result()[name]
+        assert r["C"] == pytest.approx(C, rel=1e-12), name
+        assert r["explained"] == pytest.approx(1.0, rel=1e-12), name
