@@ -236,6 +236,20 @@ int ssmd_graph_destroy(ssmd_graph *g);
  * momentum entries need rho/u from a step run with SSMD_STEP_MACROS. */
 int ssmd_phase_mass_f64(const ssmd_params *p, const ssmd_fields *d, double *out_host, void *stream);
 
+/* The populations the next step reads, in the reference's own layout, for
+ * code written against the reference's LBMFields f and g (LBMFields.cuh:26-85,
+ * arrival-indexed push layout, seven D3Q7 planes; no reference counterpart:
+ * there these ARE the buffers).  f_out: q planes of nx*ny*nz elements in
+ * p->pop_format, f_out[i][a] = d->f[i][a - c_i]; g_out: 7 float planes,
+ * g_out[k][a] = w_k (g_phi + c_k . o) of source cell a - c_k, formed with the
+ * FP32 operations of the phase update (so sum_k g_out[k][a] is its phi).
+ * Periodic images are folded in as the pulls fold them, so interior arrivals
+ * equal the reference's after its boundary kernels; an arrival with no source
+ * in the box is 0 (the reference keeps its t = 0 value there, which no kernel
+ * reads).  Either pointer may be NULL; neither may be one of d's buffers. */
+int ssmd_populations_to_reference(const ssmd_params *p, const ssmd_fields *d, void *f_out, float *g_out,
+                                  void *stream);
+
 /* Bit-exact storage conversions used by the kernels, evaluated on the device
  * (n elements, device pointers): float32 -> format -> float32. */
 int ssmd_quantize(int32_t pop_format, const float *in, float *out, uint64_t n, void *stream);

@@ -127,6 +127,8 @@ SIGNATURES = {
     "ssmd_graph_launch": (C.c_int, [C.c_void_p, C.POINTER(Fields), C.c_uint64, C.c_uint64, C.c_void_p]),
     "ssmd_graph_destroy": (C.c_int, [C.c_void_p]),
     "ssmd_phase_mass_f64": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.POINTER(C.c_double), C.c_void_p]),
+    "ssmd_populations_to_reference": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_void_p, C.c_void_p,
+                                                C.c_void_p]),
     "ssmd_quantize": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "ssmd_tune": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_int32, C.c_void_p, C.POINTER(C.c_int32),
                             C.POINTER(C.c_int32)]),

@@ -1413,6 +1413,59 @@ __global__ void __launch_bounds__(256) k_reduce2(const double *part, int nparts,
     }
     if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
 }
+
+// Source coordinate of arrival coordinate v along one axis for a direction
+// component c, with the periodic image folded in as the pulls fold it
+// (wrap_shift); -1 where the source lies outside the box.
+__device__ __forceinline__ int export_source(int v, int c, int n, int wrap) {
+    int s = v - c;
+    if (s < 0 || s >= n) return -1;
+    if (wrap && s == 0 && c > 0) s = wrap_shift(wrap, n);
+    if (wrap && s == n - 1 && c < 0) s = n - 1 - wrap_shift(wrap, n);
+    return s;
+}
+
+// The populations in the reference's arrival-indexed layout (LBMFields f
+// and g, LBMFields.cuh:26-85): fo[i][a] = f[i][a - c_i] in the storage
+// format, go[k][a] = w_k (g_phi + c_k . o) of source a - c_k formed with the
+// phase kernel's FP32 operations (phase_sum), 0 where an arrival has no
+// source in the box.  One thread per arrival cell.
+template <int Q, class P>
+__global__ void __launch_bounds__(256) k_export(KParams k, const typename P::T *__restrict__ f,
+                                                const float *__restrict__ g, typename P::T *__restrict__ fo,
+                                                float *__restrict__ go) {
+    using PT = typename P::T;
+    const uint32_t a = blockIdx.x * 256u + threadIdx.x;
+    if (a >= k.N) return;
+    const int x = int(a % uint32_t(k.nx)), y = int(a / uint32_t(k.nx) % uint32_t(k.ny)), z = int(a / k.sxy);
+    const size_t N = k.N;
+    auto source = [&](int cx, int cy, int cz) -> long {
+        const int sx = export_source(x, cx, k.nx, k.wrap_x), sy = export_source(y, cy, k.ny, k.wrap_y),
+                  sz = export_source(z, cz, k.nz, k.wrap_z);
+        return sx < 0 || sy < 0 || sz < 0 ? -1 : long(cell_id(k, sx, sy, sz));
+    };
+    if (fo) {
+        sfor<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr Dir d = kDir[i];
+            const long s = source(d.x, d.y, d.z);
+            fo[size_t(i) * N + a] = s < 0 ? PT(0) : f[size_t(i) * N + size_t(s)];
+        });
+    }
+    if (go) {
+        go[a] = weight7(0) * g[a];
+        const float *o[3] = {g + N, g + 2 * N, g + 3 * N};
+        for (int ax = 0; ax < 3; ++ax)
+            for (int sg = 0; sg < 2; ++sg) {
+                const int c = sg == 0 ? 1 : -1;
+                const long s = source(ax == 0 ? c : 0, ax == 1 ? c : 0, ax == 2 ? c : 0);
+                const int kk = 1 + 2 * ax + sg;  // +x, -x, +y, -y, +z, -z
+                float v = 0.f;
+                if (s >= 0) v = weight7(kk) * (sg == 0 ? g[s] + o[ax][s] : g[s] - o[ax][s]);
+                go[size_t(kk) * N + a] = v;
+            }
+    }
+}
 #endif  // SSMD_STEP_KERNELS
 
 template <class P>

@@ -649,6 +649,25 @@ int ssmd_phase_mass_f64(const ssmd_params *p, const ssmd_fields *d, double *out_
     return SSMD_OK;
 }
 
+int ssmd_populations_to_reference(const ssmd_params *p, const ssmd_fields *d, void *f_out, float *g_out,
+                                  void *stream) {
+    SSMD_TRY(ssmd_validate(p));
+    if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
+    if ((f_out && (f_out == d->f || f_out == d->f_next)) || (g_out && (g_out == d->g || g_out == d->g_next)))
+        return SSMD_EINVAL;  // the conversion is not in place
+    if (!f_out && !g_out) return SSMD_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const KParams k = make_kparams(*p);
+    const dim3 grid(unsigned((k.N + 255) / 256)), block(256);
+    SSMD_TRY(dispatch(*p, [&]<int Q, class P>() {
+        using PT = typename P::T;
+        hipLaunchKernelGGL((k_export<Q, P>), grid, block, 0, s, k, static_cast<const PT *>(d->f), d->g,
+                           static_cast<PT *>(f_out), g_out);
+        return herr(hipGetLastError());
+    }));
+    return SSMD_OK;
+}
+
 int ssmd_quantize(int32_t fmt, const float *in, float *out, uint64_t n, void *stream) {
     if (!in || !out) return SSMD_EINVAL;
     if (n == 0) return SSMD_OK;

@@ -12,6 +12,7 @@ Tolerances (FP32 kernels vs the FP64 oracle):
   * truly periodic closed box: phase mass drift <= 1e-6 over 1000 steps;
   * graph replay == eager steps, bit for bit.
 """
+import ctypes as C
 import json
 from pathlib import Path
 
@@ -132,6 +133,57 @@ def test_single_step(gpu, q, which):
         a, b = sim.pops_arrival(w), o.pops(w)
         m = interior_arrivals(a)
         assert np.nanmax(np.abs(a[m] - b[m])) < 2e-7, w
+
+
+# -- the reference's own layout ----------------------------------------------------
+@pytest.mark.parametrize("q,which,fmt", [(19, "jet", abi.POP_F32), (27, "jet", abi.POP_F32),
+                                         (19, "default", abi.POP_F32), (27, "jet", abi.POP_F16),
+                                         (19, "periodic", abi.POP_BF16)])
+def test_populations_to_reference_layout(gpu, q, which, fmt):
+    """ssmd_populations_to_reference after two steps: f in the storage
+    format and g as seven planes in the reference's arrival-indexed layout,
+    bit-identical to the shift of the source-indexed buffers (periodic images
+    folded in, as the pulls read them; 0 where an arrival has no source) and
+    to the FP32 formation of the seven phase populations; interior arrivals
+    against the FP64 oracle, which holds the reference's layout, as in the
+    one-step test.  The jet wraps x and y the reference's way, 'periodic' is
+    a truly periodic box."""
+    if which == "periodic":
+        p = cases.closed_box(20, q=q, pop_format=fmt)
+    else:
+        p = jet(q=q, perturb=0.01, seed=11) if which == "jet" else default_case(q)
+        p.pop_format = fmt
+    sim = run_gpu(p, 2)
+    n = p.cells
+    tdt = {abi.POP_F32: torch.float32, abi.POP_F16: torch.float16, abi.POP_BF16: torch.bfloat16}[fmt]
+    f_out = torch.full((q * n,), float("nan"), dtype=tdt, device=gpu)
+    g_out = torch.full((7 * n,), float("nan"), dtype=torch.float32, device=gpu)
+    lib = abi.load()
+    torch.cuda.synchronize()
+    abi.check(lib.ssmd_populations_to_reference(C.byref(sim.p), C.byref(sim.d), f_out.data_ptr(), g_out.data_ptr(),
+                                                sim.stream_handle), "populations_to_reference")
+    sim.synchronize()
+    f_ref = f_out.float().cpu().numpy().astype(np.float64).reshape((q,) + p.shape)
+    g_ref = g_out.cpu().numpy().astype(np.float64).reshape((7,) + p.shape)
+    for got, w in ((f_ref, "f"), (g_ref, "g")):
+        want = sim.pops_arrival(w)
+        has = ~np.isnan(want)
+        np.testing.assert_array_equal(got[has], want[has])
+        assert not got[~has].any() and has.any()
+    if which != "periodic":
+        o = run_oracle(p, 2)
+        for got, w in ((f_ref, "f"), (g_ref, "g")):
+            b = o.pops(w)
+            m = interior_arrivals(got)
+            tol = 2e-7 if fmt == abi.POP_F32 else 2e-2 * np.abs(b[m]).max()
+            assert np.abs(got[m] - b[m]).max() < tol, w
+    # arguments: nothing to do, and no in-place conversion
+    assert lib.ssmd_populations_to_reference(C.byref(sim.p), C.byref(sim.d), None, None, None) == 0
+    assert lib.ssmd_populations_to_reference(C.byref(sim.p), C.byref(sim.d), sim.d.f, None, None) == abi.EINVAL
+    # the next phase update sums these seven arrivals
+    sim.launch_phase()
+    inner = (slice(1, -1),) * 3
+    np.testing.assert_allclose(g_ref.sum(axis=0)[inner], sim.field("phi")[inner], rtol=0, atol=1e-6)
 
 
 # -- many steps ------------------------------------------------------------------
