@@ -523,7 +523,7 @@ def main() -> int:
             "box": [case.nx, case.ny, case.nz],
             "tau": case.tau, "tau_g": case.tau_g, "sigma": case.sigma, "inlet_diameter": case.diameter,
             "u_inlet": case.u, "perturb": case.perturb,
-            "timed_region": "HIP graph replay, one graph per step (phase, fused lattice, boundary kernels)",
+            "timed_region": "HIP graph replay, 16-step graphs of (phase, fused lattice, boundary kernels)",
             "step_window": f"steps {args.warmup + 3}..{args.warmup + 2 + args.steps} after t = 0 "
                            f"(2 tuning steps, {args.warmup} warm-up); throughput does not depend on the flow state",
             "lattice_tile": main_m["tile"],
