@@ -281,8 +281,10 @@ def test_learned_flux_closure_matches_oracle(gpu, q):
     assert rel_l2(o0.field("phi"), o.field("phi")) > 1e-2
 
 
-def test_learned_closure_graph_replay_matches_eager(gpu):
+@pytest.mark.parametrize("flux_coeff", [0.0, 30.0])
+def test_learned_closure_graph_replay_matches_eager(gpu, flux_coeff):
     p = learned_case()
+    p.flux_coeff = flux_coeff
     net = Network(random_params(31, bias=0.2), scales=Scales(200.0, 4.0, 0.02))
     runs = []
     for graph in (False, True):

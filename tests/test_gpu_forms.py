@@ -249,6 +249,17 @@ def test_checkpoint_rejects_another_case(gpu, tmp_path):
     r = p.copy()
     r.tile, r.lz = 3, 8
     Simulation(r).restore(tmp_path / "s.ckpt")
+    # a LEARNED checkpoint: another flux closure coefficient is another case
+    pl = p.copy()
+    pl.closure, pl.tau_scale = abi.CLOSURE_LEARNED, 0.01
+    a = Simulation(pl)
+    a.init()
+    a.checkpoint(tmp_path / "l.ckpt")
+    r = pl.copy()
+    r.flux_coeff = 2.0
+    with pytest.raises(abi.SsmdError):
+        Simulation(r).restore(tmp_path / "l.ckpt")
+    Simulation(pl).restore(tmp_path / "l.ckpt")
 
 
 # -- derived statistics ---------------------------------------------------------
