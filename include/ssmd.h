@@ -224,7 +224,10 @@ int ssmd_run(const ssmd_params *p, ssmd_fields *d, uint64_t step0, uint64_t nste
 /* Capture the step (both buffer parities) into HIP graphs, once as a single
  * step and once as a block of 16 steps.  The boundary kernels are inside the
  * graphs; the inlet-noise step index comes from d->step_counter, advanced on
- * the device by the last kernel of each step. */
+ * the device by the last kernel of each step.  With the LEARNED closure the
+ * network pass is a branch of the graph beside the boundary kernels and the
+ * next phase update (the graph owns a second, default-priority stream for
+ * the capture); results are those of ssmd_step. */
 int ssmd_graph_create(const ssmd_params *p, const ssmd_fields *d, void *stream, ssmd_graph **out);
 /* Replay nsteps steps starting at step index step0 (whole 16-step blocks,
  * then single steps); swaps *d as ssmd_step does. */

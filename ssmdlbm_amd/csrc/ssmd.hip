@@ -274,6 +274,10 @@ struct ssmd_graph {
     hipGraphExec_t block_exec[2];  // kGraphBlock steps from parity 0 / 1
     hipGraph_t block_graph[2];
     uint64_t *counter;
+    // LEARNED closure: the network pass forks onto a second stream during
+    // capture (default priority), joined before the next lattice launch
+    hipStream_t side = nullptr;
+    hipEvent_t lattice_done = nullptr, network_done = nullptr;
 };
 
 extern "C" {
@@ -568,14 +572,50 @@ int ssmd_graph_create(const ssmd_params *p, const ssmd_fields *d, void *stream, 
     g->ga = d->g;
     g->gb = d->g_next;
     g->counter = d->step_counter;
+    // The LEARNED closure's network pass reads the inputs the lattice kernel
+    // wrote and writes tau_t, which the next lattice launch reads; nothing in
+    // the boundary kernels or the next phase update touches either.  In the
+    // graphs it therefore runs as a branch beside them: forked after the
+    // lattice launch, joined before the next one (and at the graph's end).
+    const bool fork = p->closure == SSMD_CLOSURE_LEARNED;
+    if (fork) {
+        int rc = herr(hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
+        if (rc == 0) rc = herr(hipEventCreateWithFlags(&g->lattice_done, hipEventDisableTiming));
+        if (rc == 0) rc = herr(hipEventCreateWithFlags(&g->network_done, hipEventDisableTiming));
+        if (rc != 0) {
+            ssmd_graph_destroy(g);
+            return rc;
+        }
+    }
+    auto forked_step = [&](const ssmd_fields &cur, bool join_first) {
+        SSMD_TRY(launch_phase(*p, cur, s));
+        if (join_first) SSMD_HIP(hipStreamWaitEvent(s, g->network_done, 0));
+        SSMD_TRY(launch_lattice(*p, cur, 0, s));
+        SSMD_HIP(hipEventRecord(g->lattice_done, s));
+        SSMD_HIP(hipStreamWaitEvent(g->side, g->lattice_done, 0));
+        SSMD_TRY(launch_closure_update(cur.mlp, cur.feat, cur.tau_t, p->nx * p->ny * p->nz, float(p->tau_scale),
+                                       float(p->tau_floor), g->side));
+        SSMD_HIP(hipEventRecord(g->network_done, g->side));
+        return launch_boundaries(*p, cur, g->counter, 0, 0, s);
+    };
     // capture `n` steps from the buffers in `cur` (left as after those steps)
     auto capture = [&](ssmd_fields &cur, int n, hipGraph_t *graph, hipGraphExec_t *exec) {
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return int(e);
         int rc = 0;
+        bool forked = false;
         for (int i = 0; i < n && rc == 0; ++i) {
-            rc = step_dispatch(*p, cur, g->counter, 0, 0, s);
+            if (fork) {
+                rc = forked_step(cur, forked);
+                forked = true;
+            } else {
+                rc = step_dispatch(*p, cur, g->counter, 0, 0, s);
+            }
             swap_buffers(&cur);
+        }
+        if (forked) {  // join the last network pass (also after an error: capture must end joined)
+            const int rj = herr(hipStreamWaitEvent(s, g->network_done, 0));
+            if (rc == 0) rc = rj;
         }
         e = hipStreamEndCapture(s, graph);
         if (rc != 0 || e != hipSuccess) return rc != 0 ? rc : int(e);
@@ -627,6 +667,9 @@ int ssmd_graph_destroy(ssmd_graph *g) {
         if (g->block_exec[i]) (void)hipGraphExecDestroy(g->block_exec[i]);
         if (g->block_graph[i]) (void)hipGraphDestroy(g->block_graph[i]);
     }
+    if (g->lattice_done) (void)hipEventDestroy(g->lattice_done);
+    if (g->network_done) (void)hipEventDestroy(g->network_done);
+    if (g->side) (void)hipStreamDestroy(g->side);
     delete g;
     return SSMD_OK;
 }
