@@ -135,7 +135,9 @@ typedef struct ssmd_params {
      * first moments of the step get -4 D_t grad(phi), i.e. the sharpening
      * coefficient gamma phi (1 - phi) becomes gamma phi (1 - phi) - 4 D_t
      * |grad phi| along the normal.  0 (default): no flux closure; nonzero
-     * values require the LEARNED closure (else SSMD_EINVAL). */
+     * values require the LEARNED closure (else SSMD_EINVAL).  The D3Q7
+     * update stays stable (von Neumann, every 3-D mode) for D_t <= 0.375;
+     * the caller keeps flux_coeff * max(tau_t) / 3 below that. */
     double flux_coeff;
 } ssmd_params;
 
