@@ -107,9 +107,10 @@ __device__ __forceinline__ void phase_keep(float *__restrict__ phi, uint32_t c) 
 // over kPhaseLZ planes; tiles numbered x-block fastest, then y-block, then
 // z-chunk, one block per tile: ceil(nx/64) ceil((ny-2)/4) ceil((nz-2)/kPhaseLZ).
 constexpr int kPhaseLZ = 4;
-// Planes per block of k_phase_quad (28 us instead of 32 at 4 planes on a
-// 128 x 128 x 512 box; equal on 256 x 256 x 1024).
-constexpr int kPhaseQuadLZ = 8;
+// Planes per block of k_phase_quad: 16 on boxes of up to 2^24 cells, 4 above
+// (128 x 128 x 512: 27.5 us at 16 against 29.7 at 4; 256 x 256 x 1024: 276 us
+// at 4 against 289 at 16, in isolation).
+inline int phase_quad_lz(uint64_t cells) { return cells <= (uint64_t(1) << 24) ? 16 : 4; }
 
 // K1: phi = sum of the seven arriving phase populations (phaseField.cuh:31-52).
 //
@@ -200,14 +201,17 @@ __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restric
 // The phase kernel for rows of a multiple of 4 cells (every plane then starts
 // on a 16-byte boundary): a lane takes the 4 x-adjacent cells of one 16-byte
 // word of each moment plane and a block 256 consecutive such quads of the
-// interior rows (x fastest), marching over kPhaseQuadLZ planes as
-// phase_column does, with blocks dealt to the XCDs in contiguous ranges.  Per plane a lane
+// interior rows (x fastest), marching over LZ planes as phase_column does,
+// with blocks dealt to the XCDs in contiguous ranges.  Per plane a lane
 // loads five 16-byte words (g_phi, o_x of its row; g_phi, o_y of the rows
 // y -+ 1; g_phi, o_z of the plane above -- the planes below and its own are
 // carried) and the two 4-byte ends x-1, x+4 of the g_phi and o_x rows, and
 // stores one 16-byte word of phi: a quarter of the memory instructions of
-// k_phase, every cell summed by phase_sum as there.  Ghost-column cells
-// (x = 0, nx-1) keep the phi they hold.
+// k_phase, every cell summed by phase_sum as there.  The loads of plane z+1
+// are issued before plane z is summed (software pipelining: 290 -> 276 us on
+// 256 x 256 x 1024).  Ghost-column cells (x = 0, nx-1) keep the phi they hold.
+// Grid: ceil(quads / 256) ceil((nz - 2) / LZ) blocks of 256.
+template <int LZ>
 __global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__restrict__ g,
                                                     float *__restrict__ phi) {
     typedef float F4 __attribute__((ext_vector_type(4)));
@@ -217,16 +221,17 @@ __global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__re
     if (q >= rowq) return;
     const uint32_t yq = q / qx;
     const int x = int(q - yq * qx) * 4, y = 1 + int(yq);
-    const int z0 = 1 + int(zc) * kPhaseQuadLZ, z1 = min(z0 + kPhaseQuadLZ, k.nz - 1);
+    const int z0 = 1 + int(zc) * LZ, z1 = min(z0 + LZ, k.nz - 1);
     const bool first = x == 0, last = x == k.nx - 4;  // quads holding a ghost column
-    // periodic images: the +x source of cell 1 (first quad) and the -x source
-    // of cell nx-2 (last quad), the y rows of y = 1 / ny-2, the z planes
-    int sxp = 0, sxm = 0, syp = 0, sym = 0;
+    // periodic x: cell 1 (first quad) pulls +x from the image of x = 0, cell
+    // nx-2 (last quad) pulls -x from the image of x = nx-1; element offset wo
+    const bool wf = k.wrap_x && first, wl = k.wrap_x && last;
+    int wo = 0;
     if (k.wrap_x) {
         const int sh = k.wrap_x == SSMD_WRAP_TRUE ? k.nx - 2 : k.nx - 3;
-        sxp = sh;
-        sxm = -sh;
+        wo = first ? sh : 3 - sh;
     }
+    int syp = 0, sym = 0;
     if (k.wrap_y) {
         const int sh = (k.wrap_y == SSMD_WRAP_TRUE ? k.ny - 2 : k.ny - 3) * k.nx;
         if (y == 1) syp = sh;
@@ -240,32 +245,53 @@ __global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__re
     const float *__restrict__ oz = g + 3 * N;
     const int sxy = int(k.sxy);
     auto ld4 = [](const float *base, int i) { return *reinterpret_cast<const F4 *>(base + i); };
+    const int r3o = -k.nx + syp, r4o = k.nx + sym;  // rows y - 1, y + 1 (periodic images)
+    // the loads of one plane: sources above (g_phi, o_z), own o_x, the x row
+    // ends, the wrap image and the y -+ 1 rows
+    struct Plane {
+        F4 ag, az, xo, g3, o3, g4, o4;
+        float gm, om, gq, oq, gw, ow;
+    };
+    auto load = [&](int z, int c) {
+        Plane l;
+        const int sa = c + sxy - (z == k.nz - 2 ? zsh : 0);
+        l.ag = ld4(gp, sa);
+        l.az = ld4(oz, sa);
+        l.xo = ld4(ox, c);
+        l.gm = gp[c - 1];
+        l.om = ox[c - 1];
+        l.gq = gp[c + 4];
+        l.oq = ox[c + 4];
+        l.gw = 0.f;
+        l.ow = 0.f;
+        if (wf || wl) {
+            l.gw = gp[c + wo];
+            l.ow = ox[c + wo];
+        }
+        l.g3 = ld4(gp, c + r3o);
+        l.o3 = ld4(oy, c + r3o);
+        l.g4 = ld4(gp, c + r4o);
+        l.o4 = ld4(oy, c + r4o);
+        return l;
+    };
     int ci = int(cell_id(k, x, y, z0));
     const int sb = ci - sxy + (z0 == 1 ? zsh : 0);
     F4 bg = ld4(gp, sb), bz = ld4(oz, sb);  // sources of the +z populations (plane below)
     F4 cg = ld4(gp, ci), cz = ld4(oz, ci);  // own cells, this plane
+    Plane cur = load(z0, ci);
     for (int z = z0; z < z1; ++z, ci += sxy) {
-        const int sa = ci + sxy - (z == k.nz - 2 ? zsh : 0);
-        const F4 ag = ld4(gp, sa), az = ld4(oz, sa);  // sources of the -z populations (plane above)
-        const F4 xo = ld4(ox, ci);
-        // x sources: elements x-1 .. x+4 of the g_phi and o_x rows
-        float G[6] = {gp[ci - 1], cg.x, cg.y, cg.z, cg.w, gp[ci + 4]};
-        float O[6] = {ox[ci - 1], xo.x, xo.y, xo.z, xo.w, ox[ci + 4]};
-        if (k.wrap_x && first) {  // cell 1 pulls +x from the image of x = 0
-            G[1] = gp[ci + sxp];
-            O[1] = ox[ci + sxp];
-        }
-        if (k.wrap_x && last) {  // cell nx-2 pulls -x from the image of x = nx-1
-            G[4] = gp[ci + 3 + sxm];
-            O[4] = ox[ci + 3 + sxm];
-        }
-        const int r3 = ci - k.nx + syp, r4 = ci + k.nx + sym;
-        const F4 g3 = ld4(gp, r3), o3 = ld4(oy, r3), g4 = ld4(gp, r4), o4 = ld4(oy, r4);
+        Plane nxt = cur;
+        if (z + 1 < z1) nxt = load(z + 1, ci + sxy);
+        // +x source of cell e: element e-1 of the row; -x source: element e+1
+        const float G1[4] = {cur.gm, wf ? cur.gw : cg.x, cg.y, cg.z};
+        const float O1[4] = {cur.om, wf ? cur.ow : cur.xo.x, cur.xo.y, cur.xo.z};
+        const float G2[4] = {cg.y, cg.z, wl ? cur.gw : cg.w, cur.gq};
+        const float O2[4] = {cur.xo.y, cur.xo.z, wl ? cur.ow : cur.xo.w, cur.oq};
         F4 out;
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-            out[e] = phase_sum(cg[e], G[e], O[e], G[e + 2], O[e + 2], g3[e], o3[e], g4[e], o4[e], bg[e], bz[e],
-                               ag[e], az[e]);
+            out[e] = phase_sum(cg[e], G1[e], O1[e], G2[e], O2[e], cur.g3[e], cur.o3[e], cur.g4[e], cur.o4[e], bg[e],
+                               bz[e], cur.ag[e], cur.az[e]);
         if (first || last) {  // the ghost column's phi is boundary data: kept
             const F4 old = ld4(phi, ci);
             if (first) out.x = old.x;
@@ -274,8 +300,9 @@ __global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__re
         *reinterpret_cast<F4 *>(phi + ci) = out;
         bg = cg;
         bz = cz;
-        cg = ag;
-        cz = az;
+        cg = cur.ag;
+        cz = cur.az;
+        cur = nxt;
     }
 }
 #endif  // SSMD_STEP_KERNELS
@@ -710,7 +737,15 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
         //     emits exactly 0 for every population and phase moment.
         if (!active) kx = ky = kz = 0.f;
         const Cell m = rest_or(active, moments<Q, P>(fr, kx, ky, kz));
-        if (z + D < ze) pull(z + D, fr);
+        // D3Q19 16-bit: the pull is unconditional (the last plane of a chunk
+        // re-reads plane ze-1 from the caches and discards it), so the register
+        // set is rewritten on every plane and the loop carries no copies of it
+        // (-14 VALU per wave and plane, 111 -> 103 VGPRs: 2% on the kernel).
+        // The other lattices run slower so (D3Q27 FP32 -6%): conditional.
+        if constexpr (Q == 19 && sizeof(PT) == 2)
+            pull(min(z + D, ze - 1), fr);
+        else if (z + D < ze)
+            pull(z + D, fr);
         const float nu = __builtin_fmaf(ph, k.visc_diff, k.visc_water);
         float tau_t = 0.f;
         const float omco = 1.0f - fast_rcp(k.closure == SSMD_CLOSURE_NONE

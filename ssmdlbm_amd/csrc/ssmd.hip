@@ -169,10 +169,14 @@ int face_axis_host(int kind) { return face_axis(kind); }
 int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
     const KParams k = make_kparams(p);
     if (p.nx % 4 == 0) {
-        // one block per 256 quads of interior rows and kPhaseQuadLZ planes
+        // one block per 256 quads of interior rows and phase_quad_lz planes
         const unsigned quads = (p.nx / 4) * (p.ny - 2);
-        const unsigned chunks = unsigned((p.nz - 2 + kPhaseQuadLZ - 1) / kPhaseQuadLZ);
-        hipLaunchKernelGGL(k_phase_quad, dim3(((quads + 255) / 256) * chunks), dim3(256), 0, s, k, d.g, d.phi);
+        const int lz = phase_quad_lz(uint64_t(p.nx) * p.ny * p.nz);
+        const dim3 grid(((quads + 255) / 256) * unsigned((p.nz - 2 + lz - 1) / lz));
+        if (lz == 16)
+            hipLaunchKernelGGL(k_phase_quad<16>, grid, dim3(256), 0, s, k, d.g, d.phi);
+        else
+            hipLaunchKernelGGL(k_phase_quad<4>, grid, dim3(256), 0, s, k, d.g, d.phi);
     } else {
         const unsigned chunks = unsigned((p.nz - 2 + kPhaseLZ - 1) / kPhaseLZ);
         // one block per 64 x 4 tile of interior rows and kPhaseLZ planes (the kernel derives the tile)

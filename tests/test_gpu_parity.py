@@ -456,20 +456,21 @@ def test_d3q27_jet_128x128x512_against_fp64(gpu):
 
 
 @pytest.mark.slow
-def test_non_temporal_g_store_box_against_fp64(gpu):
-    """A box above 10.5 M cells (160x160x416): its 16-byte-per-cell g moment
-    planes exceed the 160 MB the lattice kernel stores through the caches, so
-    the step uses the non-temporal g stores of the large boxes.  Fields within
-    1e-5 relative L2 of the FP64 oracle after 10 steps."""
-    p = cases.JET_D3Q19_SMALL.with_(nx=160, ny=160, nz=416, diameter=16, sigma=0.05).params()
-    assert p.cells * 16 > 160 * 2 ** 20
+def test_large_box_against_fp64(gpu):
+    """A box above 2^24 cells (160x160x664, 17.0 M): its 16-byte-per-cell g
+    moment planes exceed the 160 MB the lattice kernel stores through the
+    caches, so the step uses the non-temporal g stores of the large boxes, and
+    the phase kernel marches 4-plane chunks (16 on smaller boxes).  Fields
+    within 1e-5 relative L2 of the FP64 oracle after 10 steps."""
+    p = cases.JET_D3Q19_SMALL.with_(nx=160, ny=160, nz=664, diameter=16, sigma=0.05).params()
+    assert p.cells * 16 > 160 * 2 ** 20 and p.cells > 2 ** 24
     steps = 10
-    print(f"D3Q19 160x160x416: {steps} GPU steps", flush=True)
+    print(f"D3Q19 160x160x664: {steps} GPU steps", flush=True)
     sim = run_gpu(p, steps)
     print("FP64 oracle", flush=True)
     o = run_oracle(p, steps)
     errs = {name: rel_l2(sim.field(name), o.field(name)) for name in ALL}
-    print("160x160x416 D3Q19, 10 steps, relative L2 vs FP64:", errs)
+    print("160x160x664 D3Q19, 10 steps, relative L2 vs FP64:", errs)
     for name, e in errs.items():
         assert e < (1e-4 if name.startswith("fs") else 1e-5), (name, e)
     sim.close()
