@@ -208,7 +208,9 @@ int ssmd_step(const ssmd_params *p, ssmd_fields *d, uint64_t step, uint32_t flag
  *   ssmd_launch_lattice     computeForces + computeMoments + streamCollide
  *                           (reads d->f, phi; writes d->f_next, d->g_next)
  *   ssmd_launch_boundaries  K5-K9 and the y/z periodic copies on d->f_next,
- *                           d->g_next (device step counter used if counter != 0;
+ *                           d->g_next (use_counter != 0: the inlet noise takes
+ *                           the step index from d->step_counter and the call
+ *                           advances it by one;
  *                           SSMD_STEP_MACROS in flags also refreshes rho/u on
  *                           the ghost layer, as ssmd_step does on macro steps)
  * A step run as these calls plus the LEARNED closure's ssmd_closure_update is
@@ -226,7 +228,10 @@ int ssmd_run(const ssmd_params *p, ssmd_fields *d, uint64_t step0, uint64_t nste
 /* Capture the step (both buffer parities) into HIP graphs, once as a single
  * step and once as a block of 16 steps.  The boundary kernels are inside the
  * graphs; the inlet-noise step index comes from d->step_counter, advanced on
- * the device by the last kernel of each step.  With the LEARNED closure the
+ * the device by the phase kernel that opens each step (ssmd_graph_launch sets
+ * it to step0 - 1 first).  The face kernels of a captured step among {z
+ * inlet, z outflow, x-y periodic ghost ring} run as one launch, with the
+ * results of the separate kernels.  With the LEARNED closure the
  * network pass is a branch of the graph beside the boundary kernels and the
  * next phase update (the graph owns a second, default-priority stream for
  * the capture); results are those of ssmd_step. */

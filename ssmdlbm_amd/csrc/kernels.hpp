@@ -182,7 +182,11 @@ __device__ __forceinline__ void phase_column(const KParams &k, const float *__re
 
 #ifdef SSMD_STEP_KERNELS
 // One block per phase tile, dealt to the XCDs in contiguous ranges.
-__global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi) {
+// `counter` (graph steps; else null): the device step index, advanced here,
+// at the start of a step, for the inlet noise of that step's face kernels.
+__global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restrict__ g, float *__restrict__ phi,
+                                               uint64_t *counter) {
+    if (counter && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *counter += 1;
     const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t bx = uint32_t(k.nx + 63) >> 6, by = uint32_t(k.ny - 2 + 3) >> 2;
     const uint32_t row = t / bx;
@@ -210,11 +214,13 @@ __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restric
 // k_phase, every cell summed by phase_sum as there.  The loads of plane z+1
 // are issued before plane z is summed (software pipelining: 290 -> 276 us on
 // 256 x 256 x 1024).  Ghost-column cells (x = 0, nx-1) keep the phi they hold.
-// Grid: ceil(quads / 256) ceil((nz - 2) / LZ) blocks of 256.
+// Grid: ceil(quads / 256) ceil((nz - 2) / LZ) blocks of 256.  `counter` as
+// for k_phase.
 template <int LZ>
 __global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__restrict__ g,
-                                                    float *__restrict__ phi) {
+                                                    float *__restrict__ phi, uint64_t *counter) {
     typedef float F4 __attribute__((ext_vector_type(4)));
+    if (counter && blockIdx.x == 0 && threadIdx.x == 0) *counter += 1;
     const uint32_t qx = uint32_t(k.nx) >> 2, rowq = qx * uint32_t(k.ny - 2), pb = (rowq + 255u) >> 8;
     const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t zc = t / pb, q = (t - zc * pb) * 256u + threadIdx.x;
@@ -1200,22 +1206,25 @@ __device__ __forceinline__ void periodic_face(const KParams &k, float *__restric
 // columns wrapped on both axes, since periodicY copies the row periodicX has
 // just filled.  Every source is read here and written by no thread of this
 // pass, so all cells are written in one pass with the sequential result.
+// `sb` is the plane the sources are read from: b, or nz-2 for b = nz-1 when
+// the z outflow's copy of plane nz-2 into nz-1 runs in the same launch
+// (k_faces_z); the sources are interior (x, y), which that copy covers.
 __device__ __forceinline__ void periodic_xy(const KParams &k, float *__restrict__ phi, const Macro &mac, int a, int b,
-                                            bool with_u) {
+                                            bool with_u, int sb) {
     if (a >= k.ny + k.nx - 2 || b >= k.nz) return;
     uint32_t gL, gR, sL, sR;
     if (a < k.ny) {
         const int ys = a == 0 ? k.ny - 2 : (a == k.ny - 1 ? 1 : a);  // the row periodicY leaves
         gL = cell_id(k, 0, a, b);
         gR = cell_id(k, k.nx - 1, a, b);
-        sL = cell_id(k, k.nx - 2, ys, b);
-        sR = cell_id(k, 1, ys, b);
+        sL = cell_id(k, k.nx - 2, ys, sb);
+        sR = cell_id(k, 1, ys, sb);
     } else {
         const int x = a - k.ny + 1;
         gL = cell_id(k, x, 0, b);
         gR = cell_id(k, x, k.ny - 1, b);
-        sL = cell_id(k, x, k.ny - 2, b);
-        sR = cell_id(k, x, 1, b);
+        sL = cell_id(k, x, k.ny - 2, sb);
+        sR = cell_id(k, x, 1, sb);
     }
     phi[gL] = phi[sL];
     phi[gR] = phi[sR];
@@ -1286,7 +1295,7 @@ __device__ __forceinline__ void face_op(const KParams &k, typename P::T *fout, f
     if constexpr (KIND == kPeriodicYTrue) periodic_face<1>(k, phi, mac, a, b, macros);
     if constexpr (KIND == kPeriodicZRef) periodic_face<2>(k, phi, mac, a, b, macros);
     if constexpr (KIND == kPeriodicZTrue) periodic_face<2>(k, phi, mac, a, b, macros);
-    if constexpr (KIND == kPeriodicXY) periodic_xy(k, phi, mac, a, b, macros);
+    if constexpr (KIND == kPeriodicXY) periodic_xy(k, phi, mac, a, b, macros, b);
 }
 
 // flags: bit 0 = advance the device step counter (last face kernel of the
@@ -1319,6 +1328,47 @@ __global__ void __launch_bounds__(256) k_face_batch(const JetSlot *__restrict__ 
     face_op<Q, P, KIND>(js.k, static_cast<PT *>(js.f_next), js.g_next, js.phi, js.mac, counter, step, a, b,
                         (flags & 2) != 0, int(blockIdx.z));
     if constexpr (KIND != kInflowY && KIND != kInflowZ) bump(counter, flags & 1);
+}
+
+// The face operations of a step with faces among {z inlet, z outflow, x-y
+// ghost ring} in ONE launch (graph steps, which write no ghost u), each over
+// its own range of 64x4 blocks: `nin` blocks of the inlet (face blocks times
+// its crossing-population layers), `nout` of the outflow, then the ring.
+// Run one after another, the ring copies plane nz-1 after the outflow has
+// copied plane nz-2 into it; here the ring reads plane nz-2 for those cells
+// (periodic_xy's `sb`), the values that copy writes.  The inlet disc keeps
+// clear of the x and y faces (ssmd_validate), so the inlet and the ring touch
+// disjoint cells, as do the inlet (planes 0, 1) and the outflow (nz-2, nz-1).
+// The inlet reads the device step counter; the phase kernel of the step has
+// advanced it.  Results are those of the kernels launched in order.
+template <int Q, class P, bool IN, bool OUT, bool RING>
+__global__ void __launch_bounds__(256) k_faces_z(KParams k, typename P::T *fout, float *gout, float *phi, Macro mac,
+                                                 const uint64_t *counter, uint32_t nin, uint32_t nout) {
+    uint32_t b = blockIdx.x;
+    const uint32_t fx = uint32_t(k.nx + 63) / 64, fxy = fx * (uint32_t(k.ny + 3) / 4);  // z-face blocks per layer
+    if constexpr (IN) {
+        if (b < nin) {
+            const uint32_t r = b % fxy;
+            inflow_face<Q, P, 2>(k, fout, gout, phi, mac, counter, 0, int(r % fx) * 64 + int(threadIdx.x),
+                                 int(r / fx) * 4 + int(threadIdx.y), int(b / fxy));
+            return;
+        }
+        b -= nin;
+    }
+    if constexpr (OUT) {
+        if (b < nout) {
+            const uint32_t r = b % fxy;
+            outflow_face<Q, P, 2>(k, fout, gout, phi, mac, int(r % fx) * 64 + int(threadIdx.x),
+                                  int(r / fx) * 4 + int(threadIdx.y), int(b / fxy));
+            return;
+        }
+        b -= nout;
+    }
+    if constexpr (RING) {
+        const uint32_t rx = uint32_t(k.ny + k.nx - 2 + 63) / 64;
+        const int a = int(b % rx) * 64 + int(threadIdx.x), bb = int(b / rx) * 4 + int(threadIdx.y);
+        periodic_xy(k, phi, mac, a, bb, false, OUT && bb == k.nz - 1 ? k.nz - 2 : bb);
+    }
 }
 
 #ifdef SSMD_STEP_KERNELS

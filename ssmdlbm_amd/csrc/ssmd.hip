@@ -166,7 +166,9 @@ void face_dispatch(int kind, Fn &&fn) {
 
 int face_axis_host(int kind) { return face_axis(kind); }
 
-int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
+// `counter` (graph steps): the device step index, which the phase kernel
+// advances at the start of each step (see launch_boundaries).
+int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s, uint64_t *counter = nullptr) {
     const KParams k = make_kparams(p);
     if (p.nx % 4 == 0) {
         // one block per 256 quads of interior rows and phase_quad_lz planes
@@ -174,14 +176,14 @@ int launch_phase(const ssmd_params &p, const ssmd_fields &d, hipStream_t s) {
         const int lz = phase_quad_lz(uint64_t(p.nx) * p.ny * p.nz);
         const dim3 grid(((quads + 255) / 256) * unsigned((p.nz - 2 + lz - 1) / lz));
         if (lz == 16)
-            hipLaunchKernelGGL(k_phase_quad<16>, grid, dim3(256), 0, s, k, d.g, d.phi);
+            hipLaunchKernelGGL(k_phase_quad<16>, grid, dim3(256), 0, s, k, d.g, d.phi, counter);
         else
-            hipLaunchKernelGGL(k_phase_quad<4>, grid, dim3(256), 0, s, k, d.g, d.phi);
+            hipLaunchKernelGGL(k_phase_quad<4>, grid, dim3(256), 0, s, k, d.g, d.phi, counter);
     } else {
         const unsigned chunks = unsigned((p.nz - 2 + kPhaseLZ - 1) / kPhaseLZ);
         // one block per 64 x 4 tile of interior rows and kPhaseLZ planes (the kernel derives the tile)
         const unsigned tiles = (unsigned(p.nx + 63) / 64) * (unsigned(p.ny - 2 + 3) / 4) * chunks;
-        hipLaunchKernelGGL(k_phase, dim3(tiles), dim3(64, 4), 0, s, k, d.g, d.phi);
+        hipLaunchKernelGGL(k_phase, dim3(tiles), dim3(64, 4), 0, s, k, d.g, d.phi, counter);
     }
     return herr(hipGetLastError());
 }
@@ -212,15 +214,59 @@ int launch_lattice(const ssmd_params &p, const ssmd_fields &d, uint32_t flags, h
     return launch_lattice_sel(p, d, flags, tile_of(p), p.lz, s);
 }
 
+// The face kernels of a step, in order.  `counter` null: the inlet noise
+// takes `step`.  Otherwise the inlets read the device step index there, and
+//   graph = false: the last face kernel that does not read it (or k_bump)
+//                  advances it, so a caller may run step after step;
+//   graph = true:  the step's phase kernel has advanced it already (graphs
+//                  launch the phase kernel with the counter), and on steps that
+//                  write no ghost u the faces among {z inlet, z outflow, x-y
+//                  ring} run as one launch (k_faces_z), the same results as
+//                  the kernels in order.
 int launch_boundaries(const ssmd_params &p, const ssmd_fields &d, uint64_t *counter, uint64_t step, uint32_t sflags,
-                      hipStream_t s) {
+                      hipStream_t s, bool graph = false) {
     const KParams k = make_kparams(p);
     const Macro m = make_macro(d);
     int kinds[8];
     const int nk = face_kinds(p, kinds);
+    const bool macros = (sflags & SSMD_STEP_MACROS) != 0;
+    if (graph && !macros && nk >= 2) {
+        bool in = false, out = false, ring = false, other = false;
+        for (int i = 0; i < nk; ++i) {
+            in = in || kinds[i] == kInflowZ;
+            out = out || kinds[i] == kOutflowZ;
+            ring = ring || kinds[i] == kPeriodicXY;
+            other = other || (kinds[i] != kInflowZ && kinds[i] != kOutflowZ && kinds[i] != kPeriodicXY);
+        }
+        if (!other) {
+            return dispatch(p, [&]<int Q, class P>() {
+                using PT = typename P::T;
+                PT *fo = static_cast<PT *>(d.f_next);
+                const uint32_t zface = uint32_t((k.nx + 63) / 64) * uint32_t((k.ny + 3) / 4);
+                const uint32_t nin = in ? zface * uint32_t(crossing_count<Q>()) : 0u;
+                const uint32_t nout = out ? zface * uint32_t(crossing_count<Q>()) : 0u;
+                const uint32_t nring = ring ? uint32_t((k.ny + k.nx - 2 + 63) / 64) * uint32_t((k.nz + 3) / 4) : 0u;
+                const dim3 grid(nin + nout + nring), block(64, 4);
+                if (in && out && ring)
+                    hipLaunchKernelGGL((k_faces_z<Q, P, true, true, true>), grid, block, 0, s, k, fo, d.g_next, d.phi,
+                                       m, counter, nin, nout);
+                else if (in && out)
+                    hipLaunchKernelGGL((k_faces_z<Q, P, true, true, false>), grid, block, 0, s, k, fo, d.g_next,
+                                       d.phi, m, counter, nin, nout);
+                else if (in && ring)
+                    hipLaunchKernelGGL((k_faces_z<Q, P, true, false, true>), grid, block, 0, s, k, fo, d.g_next,
+                                       d.phi, m, counter, nin, nout);
+                else
+                    hipLaunchKernelGGL((k_faces_z<Q, P, false, true, true>), grid, block, 0, s, k, fo, d.g_next,
+                                       d.phi, m, counter, nin, nout);
+                return herr(hipGetLastError());
+            });
+        }
+    }
     int last = -1;  // last kind that may advance the step counter
-    for (int i = 0; i < nk; ++i)
-        if (!is_inflow(kinds[i])) last = i;
+    if (!graph)
+        for (int i = 0; i < nk; ++i)
+            if (!is_inflow(kinds[i])) last = i;
     return dispatch(p, [&]<int Q, class P>() {
         using PT = typename P::T;
         PT *fo = static_cast<PT *>(d.f_next);
@@ -229,22 +275,24 @@ int launch_boundaries(const ssmd_params &p, const ssmd_fields &d, uint64_t *coun
             face_extent(face_axis_host(kinds[i]), k.nx, k.ny, k.nz, na, nb);
             const dim3 grid((na + 63) / 64, (nb + 3) / 4, face_layers<Q>(kinds[i]));
             face_dispatch(kinds[i], [&]<int KIND>() {
-                const int fl = int(i == last) | ((sflags & SSMD_STEP_MACROS) ? 2 : 0);
+                const int fl = int(i == last) | (macros ? 2 : 0);
                 hipLaunchKernelGGL((k_face<Q, P, KIND>), grid, dim3(64, 4), 0, s, k, fo, d.g_next, d.phi, m, counter,
                                    step, fl);
             });
         }
-        if (counter && last < 0 && nk > 0) hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, counter);
+        if (!graph && counter && last < 0 && nk > 0) hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, counter);
         return herr(hipGetLastError());
     });
 }
 
 // One step on the buffers as they are in *d (no swap).
+// `graph`: a captured step, whose phase kernel advances the device step
+// counter (launch_boundaries).
 int step_dispatch(const ssmd_params &p, const ssmd_fields &d, uint64_t *counter, uint64_t step, uint32_t flags,
-                  hipStream_t s) {
-    SSMD_TRY(launch_phase(p, d, s));
+                  hipStream_t s, bool graph = false) {
+    SSMD_TRY(launch_phase(p, d, s, graph ? counter : nullptr));
     SSMD_TRY(launch_lattice(p, d, flags, s));
-    SSMD_TRY(launch_boundaries(p, d, counter, step, flags, s));
+    SSMD_TRY(launch_boundaries(p, d, counter, step, flags, s, graph));
     if (p.closure == SSMD_CLOSURE_LEARNED)
         return launch_closure_update(d.mlp, d.feat, d.tau_t, p.nx * p.ny * p.nz, float(p.tau_scale),
                                      float(p.tau_floor), s);
@@ -592,7 +640,7 @@ int ssmd_graph_create(const ssmd_params *p, const ssmd_fields *d, void *stream, 
         }
     }
     auto forked_step = [&](const ssmd_fields &cur, bool join_first) {
-        SSMD_TRY(launch_phase(*p, cur, s));
+        SSMD_TRY(launch_phase(*p, cur, s, g->counter));
         if (join_first) SSMD_HIP(hipStreamWaitEvent(s, g->network_done, 0));
         SSMD_TRY(launch_lattice(*p, cur, 0, s));
         SSMD_HIP(hipEventRecord(g->lattice_done, s));
@@ -600,7 +648,7 @@ int ssmd_graph_create(const ssmd_params *p, const ssmd_fields *d, void *stream, 
         SSMD_TRY(launch_closure_update(cur.mlp, cur.feat, cur.tau_t, p->nx * p->ny * p->nz, float(p->tau_scale),
                                        float(p->tau_floor), g->side));
         SSMD_HIP(hipEventRecord(g->network_done, g->side));
-        return launch_boundaries(*p, cur, g->counter, 0, 0, s);
+        return launch_boundaries(*p, cur, g->counter, 0, 0, s, true);
     };
     // capture `n` steps from the buffers in `cur` (left as after those steps)
     auto capture = [&](ssmd_fields &cur, int n, hipGraph_t *graph, hipGraphExec_t *exec) {
@@ -613,7 +661,7 @@ int ssmd_graph_create(const ssmd_params *p, const ssmd_fields *d, void *stream, 
                 rc = forked_step(cur, forked);
                 forked = true;
             } else {
-                rc = step_dispatch(*p, cur, g->counter, 0, 0, s);
+                rc = step_dispatch(*p, cur, g->counter, 0, 0, s, true);
             }
             swap_buffers(&cur);
         }
@@ -650,7 +698,9 @@ int ssmd_graph_launch(ssmd_graph *g, ssmd_fields *d, uint64_t step0, uint64_t ns
         par = 1;
     else
         return SSMD_EINVAL;
-    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, s, g->counter, step0);
+    // each captured step's phase kernel advances the counter before its face
+    // kernels read it: start one below step0 (modulo 2^64)
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(1), 0, s, g->counter, step0 - 1);
     SSMD_HIP(hipGetLastError());
     uint64_t i = 0;
     for (; i + kGraphBlock <= nsteps; i += kGraphBlock)  // ends on the same parity
