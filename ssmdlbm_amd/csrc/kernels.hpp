@@ -216,15 +216,12 @@ __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restric
 // 256 x 256 x 1024).  Ghost-column cells (x = 0, nx-1) keep the phi they hold.
 // Grid: ceil(quads / 256) ceil((nz - 2) / LZ) blocks of 256.  `counter` as
 // for k_phase.
+// Quad q (< (nx/4)(ny-2), x fastest) of the interior rows, planes of z-chunk zc.
 template <int LZ>
-__global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__restrict__ g,
-                                                    float *__restrict__ phi, uint64_t *counter) {
+__device__ __forceinline__ void phase_quad_column(const KParams &k, const float *__restrict__ g,
+                                                  float *__restrict__ phi, uint32_t q, uint32_t zc) {
     typedef float F4 __attribute__((ext_vector_type(4)));
-    if (counter && blockIdx.x == 0 && threadIdx.x == 0) *counter += 1;
-    const uint32_t qx = uint32_t(k.nx) >> 2, rowq = qx * uint32_t(k.ny - 2), pb = (rowq + 255u) >> 8;
-    const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
-    const uint32_t zc = t / pb, q = (t - zc * pb) * 256u + threadIdx.x;
-    if (q >= rowq) return;
+    const uint32_t qx = uint32_t(k.nx) >> 2;
     const uint32_t yq = q / qx;
     const int x = int(q - yq * qx) * 4, y = 1 + int(yq);
     const int z0 = 1 + int(zc) * LZ, z1 = min(z0 + LZ, k.nz - 1);
@@ -310,6 +307,16 @@ __global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__re
         cz = cur.az;
         cur = nxt;
     }
+}
+template <int LZ>
+__global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__restrict__ g,
+                                                    float *__restrict__ phi, uint64_t *counter) {
+    if (counter && blockIdx.x == 0 && threadIdx.x == 0) *counter += 1;
+    const uint32_t rowq = (uint32_t(k.nx) >> 2) * uint32_t(k.ny - 2), pb = (rowq + 255u) >> 8;
+    const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
+    const uint32_t zc = t / pb, q = (t - zc * pb) * 256u + threadIdx.x;
+    if (q >= rowq) return;
+    phase_quad_column<LZ>(k, g, phi, q, zc);
 }
 #endif  // SSMD_STEP_KERNELS
 
@@ -949,6 +956,22 @@ __global__ void __launch_bounds__(256) k_phase_batch(const JetSlot *__restrict__
         return;
     }
     phase_column(k, js.g, js.phi, x, y, z0, z1);
+}
+
+// K1 for every jet when every row length is a multiple of 4: k_phase_quad's
+// quads, jet j owning blocks [cell_block_begin_j, cell_block_begin_{j+1}),
+// numbered quad block fastest, then z-chunk of kEnsemblePhaseLZ planes,
+// ceil((nx/4)(ny-2) / 256) ceil((nz-2) / kEnsemblePhaseLZ) per jet, in launch
+// order.
+constexpr int kEnsemblePhaseLZ = 4;
+__global__ void __launch_bounds__(256) k_phase_quad_batch(const JetSlot *__restrict__ jets, int njets) {
+    const uint32_t b = blockIdx.x;
+    const JetSlot &js = jets[find_slot(jets, njets, b, 1)];
+    const KParams &k = js.k;
+    const uint32_t rowq = (uint32_t(k.nx) >> 2) * uint32_t(k.ny - 2), pb = (rowq + 255u) >> 8;
+    const uint32_t lb = b - js.cell_block_begin, zc = lb / pb, q = (lb - zc * pb) * 256u + threadIdx.x;
+    if (q >= rowq) return;
+    phase_quad_column<kEnsemblePhaseLZ>(k, js.g, js.phi, q, zc);
 }
 #endif  // SSMD_STEP_KERNELS
 

@@ -862,6 +862,7 @@ struct ssmd_ensemble {
     int nk = 0, last = -1;
     JetSlot *table[2] = {nullptr, nullptr};
     uint32_t total_tiles = 0, phase_blocks = 0, face_blocks[4] = {0, 0, 0, 0};
+    bool phase_quad = false;  // every member's rows a multiple of 4: k_phase_quad_batch
     int parity = 0;
     uint64_t *counter = nullptr;
     uint32_t *queue = nullptr;  // tile queue of the persistent lattice kernel (2 words, zero between launches)
@@ -886,6 +887,8 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
     std::vector<JetSlot> h[2];
     uint32_t tiles = 0, pblocks = 0, faces[4] = {0, 0, 0, 0};
     const TileShape ts = kTiles[tile];
+    bool quad = true;
+    for (int j = 0; j < e->n; ++j) quad = quad && e->p[j].nx % 4 == 0;
     for (int j = 0; j < e->n; ++j) {
         const ssmd_params &pp = e->p[j];
         const ssmd_fields &o = e->orig[j];
@@ -900,7 +903,11 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
         js.tile_begin = tiles;
         tiles += uint32_t(js.tiles_x) * uint32_t(js.tiles_y) * uint32_t(tz);
         js.cell_block_begin = pblocks;
-        pblocks += ((js.k.sxy + 255u) / 256u) * uint32_t((pp.nz - 2 + kPhaseLZ - 1) / kPhaseLZ);
+        if (quad)
+            pblocks += ((uint32_t(pp.nx / 4) * (pp.ny - 2) + 255u) / 256u) *
+                       uint32_t((pp.nz - 2 + kEnsemblePhaseLZ - 1) / kEnsemblePhaseLZ);
+        else
+            pblocks += ((js.k.sxy + 255u) / 256u) * uint32_t((pp.nz - 2 + kPhaseLZ - 1) / kPhaseLZ);
         for (int ax = 0; ax < 4; ++ax) {
             int na, nb;
             face_extent(ax, js.k.nx, js.k.ny, js.k.nz, na, nb);
@@ -920,6 +927,7 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
         SSMD_HIP(hipMemcpy(dst[par], h[par].data(), sizeof(JetSlot) * size_t(e->n), hipMemcpyHostToDevice));
     *total_tiles = tiles;
     e->phase_blocks = pblocks;
+    e->phase_quad = quad;
     for (int ax = 0; ax < 4; ++ax) e->face_blocks[ax] = faces[ax];
     return SSMD_OK;
 }
@@ -927,7 +935,10 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
 int ensemble_launch_step(ssmd_ensemble *e, const JetSlot *tab, int tile, uint32_t total_tiles, uint64_t *counter,
                          uint64_t step, uint32_t flags, hipStream_t s) {
     const ssmd_params &p0 = e->p[0];
-    hipLaunchKernelGGL(k_phase_batch, dim3(e->phase_blocks), dim3(256), 0, s, tab, e->n);
+    if (e->phase_quad)
+        hipLaunchKernelGGL(k_phase_quad_batch, dim3(e->phase_blocks), dim3(256), 0, s, tab, e->n);
+    else
+        hipLaunchKernelGGL(k_phase_batch, dim3(e->phase_blocks), dim3(256), 0, s, tab, e->n);
     SSMD_HIP(hipGetLastError());
     const bool macros = (flags & SSMD_STEP_MACROS) != 0;
     if (macros && tile != 0) return SSMD_EINVAL;  // tables are laid out for the tile in use
@@ -1065,7 +1076,10 @@ int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *be
     int bt = e->tile;
     const ssmd_params &p0 = e->p[0];
     if (rc == 0) {
-        hipLaunchKernelGGL(k_phase_batch, dim3(e->phase_blocks), dim3(256), 0, s, e->table[e->parity], e->n);
+        if (e->phase_quad)
+            hipLaunchKernelGGL(k_phase_quad_batch, dim3(e->phase_blocks), dim3(256), 0, s, e->table[e->parity], e->n);
+        else
+            hipLaunchKernelGGL(k_phase_batch, dim3(e->phase_blocks), dim3(256), 0, s, e->table[e->parity], e->n);
         rc = herr(hipGetLastError());
     }
     for (int t = 0; t < kNumTiles && rc == 0; ++t) {
