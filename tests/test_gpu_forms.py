@@ -22,11 +22,16 @@ def small_ensemble(n=5, q=19, pop=abi.POP_F32):
     return out
 
 
-@pytest.mark.parametrize("q,pop", [(19, abi.POP_F32), (27, abi.POP_F32), (19, abi.POP_F16), (27, abi.POP_BF16)])
+@pytest.mark.parametrize("q,pop,odd", [(19, abi.POP_F32, False), (27, abi.POP_F32, False), (19, abi.POP_F16, False),
+                                       (27, abi.POP_BF16, False), (19, abi.POP_F32, True)])
 @pytest.mark.parametrize("graph", [False, True])
-def test_ensemble_members_bitwise_equal_solo(gpu, q, pop, graph):
-    # 16-bit D3Q27 runs the persistent kernel with two planes of pulls in flight
+def test_ensemble_members_bitwise_equal_solo(gpu, q, pop, odd, graph):
+    # 16-bit D3Q27 runs the persistent kernel with two planes of pulls in flight;
+    # rows of a multiple of 4 in every member run the quad phase kernel, one
+    # odd row length (`odd`) the one-cell batched phase kernel
     members = small_ensemble(q=q, pop=pop)
+    if odd:
+        members[1] = members[1].with_(nx=members[1].nx + 3)
     ens = Ensemble([c.params() for c in members])
     ens.init()
     steps = 30
