@@ -216,8 +216,10 @@ __global__ void __launch_bounds__(256) k_phase(KParams k, const float *__restric
 // 256 x 256 x 1024).  Ghost-column cells (x = 0, nx-1) keep the phi they hold.
 // Grid: ceil(quads / 256) ceil((nz - 2) / LZ) blocks of 256.  `counter` as
 // for k_phase.
-// Quad q (< (nx/4)(ny-2), x fastest) of the interior rows, planes of z-chunk zc.
-template <int LZ>
+// Quad q (< (nx/4)(ny-2), x fastest) of the interior rows, planes of z-chunk
+// zc.  NTS: phi stored non-temporally (boxes whose phi exceeds the Infinity
+// Cache: 3.7% off the kernel on 256 x 256 x 1024).
+template <int LZ, bool NTS>
 __device__ __forceinline__ void phase_quad_column(const KParams &k, const float *__restrict__ g,
                                                   float *__restrict__ phi, uint32_t q, uint32_t zc) {
     typedef float F4 __attribute__((ext_vector_type(4)));
@@ -300,7 +302,10 @@ __device__ __forceinline__ void phase_quad_column(const KParams &k, const float 
             if (first) out.x = old.x;
             if (last) out.w = old.w;
         }
-        *reinterpret_cast<F4 *>(phi + ci) = out;
+        if constexpr (NTS)
+            __builtin_nontemporal_store(out, reinterpret_cast<F4 *>(phi + ci));
+        else
+            *reinterpret_cast<F4 *>(phi + ci) = out;
         bg = cg;
         bz = cz;
         cg = cur.ag;
@@ -316,7 +321,7 @@ __global__ void __launch_bounds__(256) k_phase_quad(KParams k, const float *__re
     const uint32_t t = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t zc = t / pb, q = (t - zc * pb) * 256u + threadIdx.x;
     if (q >= rowq) return;
-    phase_quad_column<LZ>(k, g, phi, q, zc);
+    phase_quad_column<LZ, LZ == 4>(k, g, phi, q, zc);  // 4-plane chunks: boxes past 2^24 cells
 }
 #endif  // SSMD_STEP_KERNELS
 
@@ -971,7 +976,7 @@ __global__ void __launch_bounds__(256) k_phase_quad_batch(const JetSlot *__restr
     const uint32_t rowq = (uint32_t(k.nx) >> 2) * uint32_t(k.ny - 2), pb = (rowq + 255u) >> 8;
     const uint32_t lb = b - js.cell_block_begin, zc = lb / pb, q = (lb - zc * pb) * 256u + threadIdx.x;
     if (q >= rowq) return;
-    phase_quad_column<kEnsemblePhaseLZ>(k, js.g, js.phi, q, zc);
+    phase_quad_column<kEnsemblePhaseLZ, false>(k, js.g, js.phi, q, zc);
 }
 #endif  // SSMD_STEP_KERNELS
 
