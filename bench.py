@@ -371,7 +371,8 @@ def profile_mode(case, steps: int, tune: bool, tile=None) -> int:
 
     sim = Simulation(case.params(), "cuda:0")
     sim.init()
-    if tile is not None:
+    fixed = tile is not None
+    if fixed:
         sim.p.tile, sim.p.lz = tile
     elif tune:
         sim.step(2)
@@ -384,7 +385,8 @@ def profile_mode(case, steps: int, tune: bool, tile=None) -> int:
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    log(f"profile mode: tile {tile}, {steps} steps of {case.name} after tuning, 3 copies of 1 GiB")
+    how = "at the given tile" if fixed else "after tuning" if tune else "at the default tile"
+    log(f"profile mode: tile {tile}, {steps} steps of {case.name} {how}, 3 copies of 1 GiB")
     return 0
 
 
