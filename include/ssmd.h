@@ -286,7 +286,10 @@ typedef struct ssmd_ensemble ssmd_ensemble;
 int ssmd_ensemble_create(const ssmd_params *params, const ssmd_fields *fields, int32_t n, ssmd_ensemble **out);
 int ssmd_ensemble_step(ssmd_ensemble *e, uint64_t step, uint32_t flags, void *stream);
 int ssmd_ensemble_graph_launch(ssmd_ensemble *e, uint64_t step0, uint64_t nsteps, void *stream);
-/* Autotune the ensemble's lattice tile shape (kernel-only timing, as ssmd_tune). */
+/* Autotune the ensemble's lattice tile shape (kernel-only timing, as ssmd_tune).
+ * Besides the shapes 0 .. ssmd_tile_count()-1, an ensemble takes the tile
+ * index ssmd_tile_count(): 128x4 blocks that run 64x8 tiles for the members
+ * whose rows a 64-wide tile covers with fewer idle lanes (nx <= 64). */
 int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *best_tile);
 int ssmd_ensemble_set_tile(ssmd_ensemble *e, int32_t tile);
 int ssmd_ensemble_fields(const ssmd_ensemble *e, int32_t member, ssmd_fields *out);

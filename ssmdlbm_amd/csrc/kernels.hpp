@@ -545,7 +545,8 @@ template <int Q, class P, int TX, int TY, bool MACROS, class MP>
 __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P::T *__restrict__ fin,
                                              typename P::T *__restrict__ fout, float *__restrict__ gout,
                                              const float *__restrict__ phi, MP macp, int x0, int y0,
-                                             int zb, int ze, LatticeSmem<TX, TY> &sm) {
+                                             int zb, int ze, LatticeSmem<TX, TY> &sm,
+                                             int thread_x = int(threadIdx.x), int thread_y = int(threadIdx.y)) {
     static_assert(TX % 64 == 0, "a wavefront must not span two rows");
     using S = LatticeSmem<TX, TY>;
     using PT = typename P::T;
@@ -555,9 +556,9 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
     auto &s_m = sm.m;
     auto &s_n = sm.n;
 
-    const int tx = threadIdx.x, tid = tx + TX * int(threadIdx.y);
+    const int tx = thread_x, tid = tx + TX * thread_y;
     // a wavefront lies in one row: make that visible to the compiler
-    const int ty = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
+    const int ty = __builtin_amdgcn_readfirstlane(thread_y);
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x >= 1 && x <= k.nx - 2 && y >= 1 && y <= k.ny - 2;
     // The ghost-column cells x = 0 and x = nx-1 of interior rows are never
@@ -879,6 +880,7 @@ struct JetSlot {
     float *phi;
     Macro mac;
     int tiles_x, tiles_y, lz;  // lattice tiling of this jet
+    int narrow;                // its tiles are 64 x 8 in a launch of 128 x 4 blocks (k_lattice_batch)
     uint32_t tile_begin;       // first lattice tile id
     uint32_t cell_block_begin; // first block id of the phase kernel (k_phase_batch)
     uint32_t face_begin[4];    // first face block id for faces normal to x, y, z and the x-y ring (64x4 blocks)
@@ -986,10 +988,21 @@ __global__ void __launch_bounds__(256) k_phase_quad_batch(const JetSlot *__restr
 // drew cheap tiles take more of them.  A block leaves when its ticket is past
 // the last tile; the last block to leave (queue[1] counts them) resets both
 // words, so every launch -- a graph replay included -- starts from 0.
+// LDS of the 128 x 4 persistent kernel, which also runs 64 x 8 tiles (the
+// same 512 threads) for members whose rows a 64-wide tile covers with fewer
+// idle lanes; 64 x 8 needs less LDS than 128 x 4.
+template <int TX, int TY>
+union MixedSmem {
+    LatticeSmem<TX, TY> main;
+    LatticeSmem<64, 8> alt;
+};
+template <int TX, int TY, bool MACROS> constexpr bool mixed_tiles() { return TX == 128 && TY == 4 && !MACROS; }
+
 template <int Q, class P, int TX, int TY, bool MACROS>
 __global__ void __launch_bounds__(TX *TY) k_lattice_batch(const JetSlot *__restrict__ jets, int njets,
                                                           uint32_t total_tiles, uint32_t *__restrict__ queue) {
-    __shared__ LatticeSmem<TX, TY> sm;
+    constexpr bool kMixed = mixed_tiles<TX, TY, MACROS>();
+    __shared__ std::conditional_t<kMixed, MixedSmem<TX, TY>, LatticeSmem<TX, TY>> sm;
     __shared__ uint32_t ticket;
     using PT = typename P::T;
     const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
@@ -1007,10 +1020,23 @@ __global__ void __launch_bounds__(TX *TY) k_lattice_batch(const JetSlot *__restr
         const int by = int(lt % uint32_t(js.tiles_y));
         const int bz = int(lt / uint32_t(js.tiles_y));
         const int zb = 1 + bz * js.lz;
-        lattice_tile<Q, P, TX, TY, MACROS>(js.k, static_cast<const PT *>(js.f), static_cast<PT *>(js.f_next),
-                                           js.g_next, js.phi,
-                                           &js.mac, bx * TX, by * TY, zb,
-                                           min(zb + js.lz, js.k.nz - 1), sm);
+        const int ze = min(zb + js.lz, js.k.nz - 1);
+        if constexpr (kMixed) {
+            if (js.narrow) {
+                // the 512 threads as 64 x 8: a wavefront (64 consecutive x of one
+                // threadIdx.y) stays one row
+                const int l = int(threadIdx.x) + TX * int(threadIdx.y);
+                lattice_tile<Q, P, 64, 8, MACROS>(js.k, static_cast<const PT *>(js.f), static_cast<PT *>(js.f_next),
+                                                  js.g_next, js.phi, &js.mac, bx * 64, by * 8, zb, ze, sm.alt,
+                                                  l & 63, l >> 6);
+            } else {
+                lattice_tile<Q, P, TX, TY, MACROS>(js.k, static_cast<const PT *>(js.f), static_cast<PT *>(js.f_next),
+                                                   js.g_next, js.phi, &js.mac, bx * TX, by * TY, zb, ze, sm.main);
+            }
+        } else {
+            lattice_tile<Q, P, TX, TY, MACROS>(js.k, static_cast<const PT *>(js.f), static_cast<PT *>(js.f_next),
+                                               js.g_next, js.phi, &js.mac, bx * TX, by * TY, zb, ze, sm);
+        }
     }
     if (lead &&
         __hip_atomic_fetch_add(&queue[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {

@@ -886,7 +886,8 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
     SSMD_HIP(hipDeviceSynchronize());  // no launch may still read the tables being replaced
     std::vector<JetSlot> h[2];
     uint32_t tiles = 0, pblocks = 0, faces[4] = {0, 0, 0, 0};
-    const TileShape ts = kTiles[tile];
+    const bool mixed = tile == kEnsembleMixedTile;
+    const TileShape ts = kTiles[mixed ? 3 : tile];
     bool quad = true;
     for (int j = 0; j < e->n; ++j) quad = quad && e->p[j].nx % 4 == 0;
     for (int j = 0; j < e->n; ++j) {
@@ -896,9 +897,11 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
         js.k = make_kparams(pp);
         js.phi = o.phi;
         js.mac = make_macro(o);
-        js.tiles_x = int((pp.nx + ts.tx - 1) / ts.tx);
-        js.tiles_y = int((pp.ny + ts.ty - 1) / ts.ty);
-        js.lz = pp.lz > 0 ? pp.lz : auto_lz(int(pp.nx), int(pp.ny), int(pp.nz), tile);
+        js.narrow = mixed && narrow_member(int(pp.nx)) ? 1 : 0;
+        const int tx = js.narrow ? 64 : ts.tx, ty = js.narrow ? 8 : ts.ty;
+        js.tiles_x = int((pp.nx + tx - 1) / tx);
+        js.tiles_y = int((pp.ny + ty - 1) / ty);
+        js.lz = pp.lz > 0 ? pp.lz : auto_lz_shape(int(pp.nx), int(pp.ny), int(pp.nz), tx, ty);
         const int tz = (int(pp.nz) - 2 + js.lz - 1) / js.lz;
         js.tile_begin = tiles;
         tiles += uint32_t(js.tiles_x) * uint32_t(js.tiles_y) * uint32_t(tz);
@@ -943,7 +946,9 @@ int ensemble_launch_step(ssmd_ensemble *e, const JetSlot *tab, int tile, uint32_
     const bool macros = (flags & SSMD_STEP_MACROS) != 0;
     if (macros && tile != 0) return SSMD_EINVAL;  // tables are laid out for the tile in use
     SSMD_TRY(dispatch(p0, [&]<int Q, class P>() {
-        return herr(launch_lattice_batch<Q, P>(tile, macros, tab, e->n, total_tiles, e->queue, s));
+        // the mixed tiling runs the 128 x 4 kernel, which reads each member's shape from its slot
+        return herr(launch_lattice_batch<Q, P>(tile == kEnsembleMixedTile ? 3 : tile, macros, tab, e->n, total_tiles,
+                                               e->queue, s));
     }));
     return dispatch(p0, [&]<int Q, class P>() {
         for (int i = 0; i < e->nk; ++i) {
@@ -1010,7 +1015,7 @@ int ssmd_ensemble_create(const ssmd_params *params, const ssmd_fields *fields, i
 }
 
 int ssmd_ensemble_set_tile(ssmd_ensemble *e, int32_t tile) {
-    if (!e || tile < 0 || tile >= kNumTiles) return SSMD_EINVAL;
+    if (!e || tile < 0 || tile > kEnsembleMixedTile) return SSMD_EINVAL;
     if (tile == e->tile) return SSMD_OK;
     ensemble_drop_graphs(e);
     e->tile = tile;
@@ -1082,14 +1087,15 @@ int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *be
             hipLaunchKernelGGL(k_phase_batch, dim3(e->phase_blocks), dim3(256), 0, s, e->table[e->parity], e->n);
         rc = herr(hipGetLastError());
     }
-    for (int t = 0; t < kNumTiles && rc == 0; ++t) {
+    for (int t = 0; t <= kEnsembleMixedTile && rc == 0; ++t) {
         uint32_t total = 0;
         rc = ensemble_tables(e, t, tmp, &total);
         if (rc != 0) break;
         float ms = 0.f;
         rc = time_launches(s, reps, [&]() {
             return dispatch(p0, [&]<int Q, class P>() {
-                return herr(launch_lattice_batch<Q, P>(t, false, tmp[e->parity], e->n, total, e->queue, s));
+                return herr(launch_lattice_batch<Q, P>(t == kEnsembleMixedTile ? 3 : t, false, tmp[e->parity], e->n,
+                                                       total, e->queue, s));
             });
         }, &ms);
         if (rc == 0 && ms < best) {

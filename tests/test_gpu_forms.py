@@ -135,7 +135,9 @@ def test_tile_shapes_agree(gpu, q, pop, nx):
 def test_ensemble_tile_shapes_agree(gpu):
     members = small_ensemble(n=3)
     res = []
-    for tile in range(abi.load().ssmd_tile_count()):
+    # the last index is the ensemble-only mixed tiling (64 x 8 tiles for the
+    # members of at most 64 cells per row: all of these)
+    for tile in range(abi.load().ssmd_tile_count() + 1):
         ens = Ensemble([c.params() for c in members])
         ens.init()
         ens.set_tile(tile)
@@ -167,7 +169,7 @@ def test_ensemble_tune(gpu):
     ens = Ensemble([c.params() for c in small_ensemble(n=3)])
     ens.init()
     t = ens.tune(reps=2)
-    assert 0 <= t < abi.load().ssmd_tile_count()
+    assert 0 <= t <= abi.load().ssmd_tile_count()  # the last: the ensemble-only mixed tiling
     ens.graph_run(3)
 
 
