@@ -437,6 +437,34 @@ def test_boundary_kernels_from_a_set_state(gpu, q, faces):
 
 
 # -- the throughput boxes themselves ---------------------------------------------------
+def test_manual_steps_on_the_device_counter(gpu):
+    """ssmd_launch_boundaries with use_counter = 1 takes the inlet noise's step
+    index from the device counter and advances it: phase, lattice and
+    boundary launches stepped that way from counter 3 equal eager ssmd_step
+    calls for steps 3, 4, ..., and the counter ends at the next step."""
+    p = jet(perturb=0.01, seed=13)
+    steps = 9
+    a = Simulation(p)
+    a.init()
+    a.step_index = 3
+    a.step(steps)
+    b = Simulation(p)
+    b.init()
+    b.t["step_counter"].fill_(3)
+    torch.cuda.synchronize()
+    for _ in range(steps):
+        b.launch_phase()
+        b.launch_lattice()
+        abi.check(b.lib.ssmd_launch_boundaries(C.byref(b.p), C.byref(b.d), 0, 1, 0, b.stream_handle), "boundaries")
+        b.swap()
+    b.synchronize()
+    assert int(b.t["step_counter"].item()) == 3 + steps
+    np.testing.assert_array_equal(a.pops_source("f"), b.pops_source("f"))
+    np.testing.assert_array_equal(a.g_moments(), b.g_moments())
+    for name in MACRO:
+        np.testing.assert_array_equal(a.field(name), b.field(name), err_msg=name)
+
+
 @pytest.mark.parametrize("faces", ["inlet+outflow+ring", "inlet+outflow", "inlet+ring", "outflow+ring"])
 @pytest.mark.parametrize("q", [19, 27])
 def test_graph_face_sets_match_eager_bitwise(gpu, q, faces):
