@@ -263,6 +263,108 @@ def test_shear_wave_decay(q):
     assert abs(amp / want - 1) < 0.02, (amp, want)
 
 
+def _dev(a):
+    return a - np.trace(a) / 3 * np.eye(3)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_guo_forcing_uniform_acceleration(q):
+    """A uniform flow under a uniform body force F (reference lbm.cu:83-85,
+    107 and 253: half the force in u and in Pi, half in the collision).  With
+    f = f_eq(u0) at rho = 1 the Guo scheme gives, in closed form,
+    u1 = u0 + F/2 and Pi1 = dev(F F)/4 after the first moments pass; one
+    collision then adds exactly F to the momentum (u2 = u1 + F) and relaxes
+    the stress as Pi2 = (1 - omega) Pi1, the force terms cancelling."""
+    L = load()
+    n, tau = 8, 0.7
+    p = _box(n, q=q, tau=tau, sigma=0.0)
+    o = Oracle(p, threads=cpu_threads())
+    o.init()
+    u0 = np.array([0.04, -0.03, 0.02])
+    F = np.array([2e-3, 1e-3, -1.5e-3])
+    f = np.array([L.oracle_feq(q, i, 1.0, *u0) for i in range(q)])
+    o.set_pops("f", f[:, None, None, None] * np.ones((q,) + o.shape))
+    for k, v in zip(("fsx", "fsy", "fsz"), F):
+        o.set_field(k, np.full(o.shape, v))
+    inner = (slice(2, -2),) * 3  # cells whose pulls all come from interior cells
+
+    def state():
+        u = np.array([o.field(k)[inner] for k in ("ux", "uy", "uz")])
+        P = {k: o.field(k)[inner] for k in ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz")}
+        Pi = np.array([[P["pxx"], P["pxy"], P["pxz"]], [P["pxy"], P["pyy"], P["pyz"]], [P["pxz"], P["pyz"], P["pzz"]]])
+        np.testing.assert_allclose(o.field("rho")[inner], 1.0, atol=1e-14)
+        return u, Pi
+
+    o.k_moments()
+    u1, Pi1 = state()
+    np.testing.assert_allclose(u1, (u0 + F / 2)[:, None, None, None] * np.ones_like(u1), atol=1e-15)
+    want1 = _dev(np.outer(F, F)) / 4
+    np.testing.assert_allclose(Pi1, want1[:, :, None, None, None] * np.ones_like(Pi1), atol=1e-15)
+    o.k_collide()
+    o.k_moments()
+    u2, Pi2 = state()
+    np.testing.assert_allclose(u2, (u0 + 3 * F / 2)[:, None, None, None] * np.ones_like(u2), atol=1e-15)
+    np.testing.assert_allclose(Pi2, (1 - 1 / tau) * want1[:, :, None, None, None] * np.ones_like(Pi2), atol=1e-15)
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.5, 1.0])
+def test_shear_wave_viscosity_follows_phase(phi):
+    """Viscosity is interpolated linearly in phi between the water and oil
+    values (reference lbm.cu:229): a shear wave in a uniform phase phi decays
+    at nu = nu_w + phi (nu_o - nu_w).  Sharpening is off (gamma = 0): it
+    would separate a uniform mixture 0 < phi < 1 from roundoff gradients."""
+    tau_w, tau_o, L, A, T = 0.6, 1.0, 32, 1e-3, 200
+    n = L + 2
+    p = _box(n, q=19, sigma=0.0, gamma=0.0)
+    p.visc_water = cases.viscosity_from_tau(tau_w)
+    p.visc_oil = cases.viscosity_from_tau(tau_o)
+    o = Oracle(p, threads=cpu_threads())
+    o.init()
+    lib = load()
+    o.set_pops("g", np.array([lib.oracle_geq(k, phi, 0, 0, 0) for k in range(7)])[:, None, None, None]
+               * np.ones((7,) + o.shape))
+    o.set_field("phi", np.full(o.shape, phi))
+    z = np.arange(n) - 1.0
+    prof0 = A * np.sin(2 * np.pi * z / L)
+    f = np.stack([np.array([lib.oracle_feq(19, i, 1.0, v, 0.0, 0.0) for v in prof0]) for i in range(19)])
+    o.set_pops("f", f[:, :, None, None] * np.ones((19,) + o.shape))
+    o.step(T)
+    o.step(1)
+    prof = o.field("ux")[1:-1, n // 2, n // 2]
+    amp = 2 / L * np.sum(prof * np.sin(2 * np.pi * z[1:-1] / L))
+    nu = (tau_w - 0.5) / 3 + phi * ((tau_o - 0.5) / 3 - (tau_w - 0.5) / 3)
+    want = A * np.exp(-nu * (2 * np.pi / L) ** 2 * T)
+    assert abs(amp / want - 1) < 0.02, (amp, want)
+    np.testing.assert_allclose(o.field("phi")[1:-1, 1:-1, 1:-1], phi, atol=1e-12)
+
+
+def test_interface_sharpening_flux():
+    """The phase collision adds the conservative Allen-Cahn sharpening flux
+    gamma phi (1 - phi) n (reference lbm.cu:256): on a planar interface at
+    rest, the first moment each cell sends out, sum_k c_k g_k, is
+    gamma phi (1 - phi) n / 4 with n = grad phi / |grad phi| along +z."""
+    lib = load()
+    n, gamma = 16, 0.3
+    p = _box(n, q=19, sigma=0.0, gamma=gamma)
+    o = Oracle(p, threads=cpu_threads())
+    o.init()
+    zz = np.arange(n, dtype=float)
+    prof = 0.5 * (1 + np.tanh((zz - (n - 1) / 2) / 3))
+    o.set_field("phi", prof[:, None, None] * np.ones(o.shape))
+    o.k_moments()
+    o.k_collide()
+    c = np.array([[lib.oracle_c(k, a) for a in range(3)] for k in range(7)])
+    kp = int(np.flatnonzero((c == [0, 0, 1]).all(1))[0])
+    km = int(np.flatnonzero((c == [0, 0, -1]).all(1))[0])
+    g = o.pops("g")
+    src = np.arange(1, n - 1)
+    sent = g[kp][src + 1] - g[km][src - 1]  # c_z-moment sent from planes 1..n-2
+    ph = prof[src]
+    want = gamma * ph * (1 - ph) / 4
+    np.testing.assert_allclose(sent[:, 1:-1, 1:-1], want[:, None, None] * np.ones_like(sent[:, 1:-1, 1:-1]),
+                               rtol=1e-6, atol=1e-15)
+
+
 def _droplet_box(q, n=20, wrap=abi.WRAP_TRUE):
     p = _box(n, q=q, sigma=0.02)
     p.wrap_x = p.wrap_y = p.wrap_z = wrap
