@@ -174,8 +174,9 @@ __device__ __forceinline__ f32x16 unit_image(const float *v, int mb, int lane) {
     return r;
 }
 
-// (4 waves per SIMD, i.e. at most 128 registers: left free, the compiler
-// parks accumulators in AGPRs and runs 1 wave per SIMD; the kernel needs 126)
+// (2 waves per SIMD: the two-tile loop holds the weights, two tiles'
+// accumulators and their B operands in 200 VGPRs; left free, the compiler
+// parks accumulators in AGPRs and runs 1 wave per SIMD)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_forward(const uint8_t *__restrict__ packed, const uint16_t *__restrict__ x,
                                                  float *__restrict__ y, uint32_t n, float scale, float floor,
