@@ -346,3 +346,54 @@ def test_f16_store_is_rne_of_the_fp32_result(gpu, q):
     assert bad == 0, f"{bad} of {got.size} f16 slots differ from RNE of the FP32 step"
     s16.close()
     s32.close()
+
+
+@pytest.mark.parametrize("fmt", [abi.POP_F16, abi.POP_BF16], ids=["f16", "bf16"])
+def test_full_size_16bit_step_is_rne_of_fp32(gpu, fmt):
+    """The 16-bit storage case at its own size (256x256x1024, 67 M cells, the
+    128x4 tile and 32-plane chunks the tuner picks there), so the paths only
+    large boxes take run: non-temporal g stores, 4-plane phase chunks with
+    non-temporal phi stores.  From one random 16-bit-representable state in
+    every slot (ghosts included), one 16-bit step stores exactly RNE of the
+    FP32 step's populations, and the two steps' g moments and phi, computed
+    in FP32 from the same values, are bit-identical."""
+    p16 = cases.JET_D3Q19_FP16.with_(pop_format=fmt).params()
+    p16.tile, p16.lz = 3, 32
+    p32 = p16.copy()
+    p32.pop_format = abi.POP_F32
+    n = p16.cells
+    print(f"256x256x1024 format {fmt}: random state, one step each", flush=True)
+    s16 = Simulation(p16)
+    s16.init()
+    s16.synchronize()
+    gen = torch.Generator(device=gpu).manual_seed(11)
+    f16 = s16._current("f")
+    f16.copy_(torch.randn(f16.numel(), generator=gen, device=gpu) * 2e-3)  # cast: RNE
+    g = s16._current("g")
+    g[:n].copy_(torch.rand(n, generator=gen, device=gpu))
+    g[n:].copy_((torch.rand(3 * n, generator=gen, device=gpu) - 0.5) * 0.02)
+    s16.t["phi"].copy_(torch.rand(n, generator=gen, device=gpu))
+    s32 = Simulation(p32)
+    s32.init()
+    s32.synchronize()
+    s32._current("f").copy_(f16.float())
+    s32._current("g").copy_(g)
+    for name, t in s16.t.items():
+        if t.dtype == torch.float32 and name in s32.t:
+            s32.t[name].copy_(t)
+    torch.cuda.synchronize()
+    s16.step(1)
+    s32.step(1)
+    s16.synchronize()
+    s32.synchronize()
+    got = s16._current("f").view(torch.int16)
+    want = s32._current("f").to(f16.dtype).view(torch.int16)  # torch: IEEE RNE
+    assert int(torch.count_nonzero(got)) > 0.9 * got.numel()
+    bad = int((got != want).sum())
+    assert bad == 0, f"{bad} of {got.numel()} 16-bit slots differ from RNE of the FP32 step"
+    assert torch.equal(s16._current("g"), s32._current("g"))
+    assert torch.equal(s16.t["phi"], s32.t["phi"])
+    s16.close()
+    s32.close()
+    del s16, s32, got, want, f16, g
+    torch.cuda.empty_cache()
