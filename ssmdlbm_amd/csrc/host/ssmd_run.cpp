@@ -156,10 +156,24 @@ int main(int argc, char **argv) {
     check(ssmd_graph_create(&p, &d, s, &g), "graph");
     const char *names[] = {"phi", "rho"};  // main.cu:126-129
 
+    // NaN/Inf sentinel (the reference has none): the FP64 sums of phi and
+    // rho u after a step that wrote every moment must be finite
+    auto sentinel = [&](uint64_t step) {
+        double sums[4];
+        check(ssmd_phase_mass_f64(&p, &d, sums, s), "sentinel");
+        for (double v : sums)
+            if (!std::isfinite(v)) {
+                std::cerr << "non-finite state at step " << step << ": sum phi " << sums[0] << ", sum rho u ("
+                          << sums[1] << ", " << sums[2] << ", " << sums[3] << ")\n";
+                std::exit(2);
+            }
+    };
+
     auto save = [&](uint64_t step) {
-        // a step that writes every moment field, then the files
+        // a step that writes every moment field, then the sentinel and the files
         check(ssmd_step(&p, &d, step, SSMD_STEP_MACROS, s), "macro step");
         check(int(hipStreamSynchronize(s)), "sync");
+        sentinel(step);
         std::cout << "Step " << step << ": bins in " << dir.string() << "\n";
         check(ssmd_write_bin(&p, d.phi, dir.c_str(), "phi", step, s), "bin phi");
         check(ssmd_write_bin(&p, d.rho, dir.c_str(), "rho", step, s), "bin rho");
@@ -193,6 +207,7 @@ int main(int argc, char **argv) {
     std::cout << "     Total execution time    : " << elapsed << " s\n";
     std::cout << "     Performance             : " << lups / (elapsed * 1e6) << " MLUPS\n";
     std::cout << "// =============================================== //\n\n";
+    sentinel(nsteps);
     double mass[4];
     check(ssmd_phase_mass_f64(&p, &d, mass, s), "mass");
     std::cout << "     Phase mass (FP64)       : " << mass[0] << "\n";

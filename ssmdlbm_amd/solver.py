@@ -212,6 +212,18 @@ class Simulation:
                   "ssmd_phase_mass_f64")
         return np.array(out[:], dtype=np.float64)
 
+    def check_finite(self) -> np.ndarray:
+        """NaN/Inf sentinel (the reference has none): the FP64 sums of
+        phase_mass() must be finite.  One non-finite population reaches rho and
+        u of the cell that pulls it in the next step, and phi one step later;
+        the momentum sums see it only when the last step ran with macros=True.
+        Returns the sums; raises SsmdError naming the step otherwise."""
+        s = self.phase_mass()
+        if not np.isfinite(s).all():
+            raise abi.SsmdError(f"non-finite state after step {self.step_index}: "
+                                f"sum phi {s[0]!r}, sum rho u ({s[1]!r}, {s[2]!r}, {s[3]!r})")
+        return s
+
 
 def _source_axis(n: int, c: int, wrap: int):
     """Source coordinate of each arrival coordinate 0..n-1 along one axis for

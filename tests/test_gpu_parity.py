@@ -351,6 +351,27 @@ def test_phase_mass_reduction(gpu):
     np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-9)
 
 
+def test_nan_sentinel_names_the_step(gpu):
+    """The NaN/Inf sentinel (Simulation.check_finite: the FP64 sums of phi and
+    rho u must be finite) passes a healthy run; one population slot of one
+    interior source cell poisoned with NaN reaches rho and u of the cell that
+    pulls it, so the sentinel raises after the next macro step, naming it."""
+    p = cases.JET_D3Q19_SMALL.with_(nx=24, ny=22, nz=30, diameter=6).params()
+    sim = Simulation(p)
+    sim.init()
+    sim.step(3, macros=True)
+    assert np.isfinite(sim.check_finite()).all()
+    sim.synchronize()
+    x, y, z = 12, 11, 15
+    cell = (z * p.ny + y) * p.nx + x
+    sim._current("f")[5 * p.cells + cell] = float("nan")
+    torch.cuda.synchronize()
+    sim.step(1, macros=True)
+    with pytest.raises(abi.SsmdError, match="non-finite state after step 4"):
+        sim.check_finite()
+    sim.close()
+
+
 # -- precision forms ----------------------------------------------------------------
 @pytest.mark.parametrize("q", [19, 27])
 @pytest.mark.parametrize("fmt,tol", [(abi.POP_F16, 2e-3), (abi.POP_BF16, 2e-2)])
