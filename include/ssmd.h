@@ -297,7 +297,9 @@ int ssmd_ensemble_destroy(ssmd_ensemble *e);
 
 /* Output in the reference's formats (host code; synchronizes `stream`):
  *   ssmd_write_bin   <dir>/<name><step:06>.bin, raw float32 of a device field
- *                    (host::PostProcess::bin, postProcess/PostProcess.cuh:45-98)
+ *                    (host::PostProcess::bin, postProcess/PostProcess.cuh:45-98);
+ *                    a null field (normx .. ind are never allocated) is
+ *                    skipped: SSMD_OK, no file (PostProcess.cuh:64-73)
  *   ssmd_write_vti   <dir>/step_<step:06>.vti from those .bin files: VTK
  *                    ImageData, appended raw, uint32 byte count per array;
  *                    missing or wrong-size .bin files are skipped

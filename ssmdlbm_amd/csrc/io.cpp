@@ -98,8 +98,9 @@ extern "C" {
 
 int ssmd_write_bin(const ssmd_params *p, const float *device_field, const char *dir, const char *name, uint64_t step,
                    void *stream) {
-    if (!p || !device_field || !dir || !name) return SSMD_EINVAL;
+    if (!p || !dir || !name) return SSMD_EINVAL;
     if (ssmd_validate(p) != 0) return SSMD_EINVAL;
+    if (!device_field) return SSMD_OK;  // a field never allocated: skipped, as PostProcess::bin does
     const size_t n = size_t(p->nx) * p->ny * p->nz;
     std::vector<float> h(n);
     hipStream_t s = static_cast<hipStream_t>(stream);

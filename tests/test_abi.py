@@ -164,6 +164,19 @@ def test_error_paths_without_device():
         abi.check(-2, "x")
 
 
+def test_bin_skips_a_null_field(tmp_path):
+    """PostProcess::bin skips a field whose device pointer is null (normx ..
+    ind are declared but never allocated; PostProcess.cuh:64-73): no file, no
+    error.  A missing directory name or parameter block is still an error."""
+    lib = abi.load()
+    p = cases.JET_D3Q19_SMALL.params()
+    d = str(tmp_path).encode()
+    assert lib.ssmd_write_bin(p, None, d, b"normx", 7, None) == 0
+    assert list(tmp_path.iterdir()) == []
+    assert lib.ssmd_write_bin(p, None, None, b"normx", 7, None) == abi.EINVAL
+    assert lib.ssmd_write_bin(None, None, d, b"normx", 7, None) == abi.EINVAL
+
+
 def test_info_file_reference_lines(tmp_path):
     """The info file of the reference's own case carries its metadata lines
     (hostFunctions.cuh:63-76): Reynolds 1400 / 450 and Weber 500 come back
