@@ -34,7 +34,11 @@
  *     1/8).  The collision emits exactly this family (o = 4 phi u +
  *     gamma phi (1-phi) n), so the arriving sum is the same FP32 sum as
  *     with seven population planes, with 16 instead of 28 bytes per cell.
- *   - Two buffer pairs ping-pong; ssmd_step swaps f<->f_next, g<->g_next.
+ *   - f ping-pongs: ssmd_step swaps f<->f_next (and g<->g_next).  g needs no
+ *     second buffer: its one reader, the phase update, runs before the
+ *     lattice kernel of the same step rewrites it, so g_next may be the same
+ *     buffer as g, as ssmd_alloc makes it (a one-step g then stays in the
+ *     Infinity Cache between the two kernels).
  */
 #ifndef SSMD_H
 #define SSMD_H
@@ -155,7 +159,7 @@ typedef struct ssmd_fields {
     void *f;          /* populations the next step reads (pop_format elements) */
     float *g;         /* D3Q7 phase populations the next step reads (4 moment planes) */
     void *f_next;     /* buffers the next step writes */
-    float *g_next;
+    float *g_next;    /* may equal g (ssmd_alloc: one g buffer) */
     uint64_t *step_counter; /* device word: inlet-noise step index for graph replays */
     double *reduce;   /* SSMD_REDUCE_DOUBLES doubles of reduction workspace */
     uint64_t owned;   /* nonzero when ssmd_alloc owns the buffers */

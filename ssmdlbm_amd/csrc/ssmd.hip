@@ -468,11 +468,13 @@ int ssmd_alloc(const ssmd_params *p, ssmd_fields *d) {
         e = hipMemset(*q, 0, fb);
         if (e != hipSuccess) return fail(e);
     }
-    float **gs[] = {&d->g, &d->g_next};
-    for (float **q : gs) {
-        hipError_t e = hipMalloc(reinterpret_cast<void **>(q), gb);
+    {
+        // one g buffer: the phase update reads g before the lattice kernel of
+        // the same step writes it (g_next), so the pair can share it
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&d->g), gb);
         if (e != hipSuccess) return fail(e);
-        e = hipMemset(*q, 0, gb);
+        d->g_next = d->g;
+        e = hipMemset(d->g, 0, gb);
         if (e != hipSuccess) return fail(e);
     }
     hipError_t e = hipMalloc(reinterpret_cast<void **>(&d->step_counter), sizeof(uint64_t));
@@ -503,11 +505,15 @@ int ssmd_free(ssmd_fields *d) {
                     d->pyz, d->phi, d->fsx, d->fsy, d->fsz, d->f,   d->f_next, d->g,           d->g_next,
                     d->step_counter, d->reduce, d->tau_t, d->feat, d->mlp};
     int rc = 0;
-    for (void *q : ptrs)
-        if (q) {
-            const int r = herr(hipFree(q));
+    const size_t np = sizeof(ptrs) / sizeof(ptrs[0]);
+    for (size_t i = 0; i < np; ++i) {
+        bool seen = false;  // g_next may be g itself: each buffer freed once
+        for (size_t j = 0; j < i; ++j) seen = seen || ptrs[j] == ptrs[i];
+        if (ptrs[i] && !seen) {
+            const int r = herr(hipFree(ptrs[i]));
             if (r && !rc) rc = r;
         }
+    }
     std::memset(d, 0, sizeof(*d));
     return rc;
 }
@@ -815,6 +821,47 @@ int time_launches(hipStream_t s, int reps, F &&launch, float *ms) {
     return rc;
 }
 
+// The tuners' trial lattice launches write g_next.  A field set whose g_next
+// is g itself (one g buffer) would lose the g the next step reads, so the
+// tuners keep a device copy of such buffers and put it back afterwards.
+struct GKeeper {
+    std::vector<std::pair<float *, size_t>> bufs;  // (g, bytes)
+    char *copy = nullptr;
+    hipStream_t s = nullptr;
+    int save(hipStream_t st) {
+        s = st;
+        size_t total = 0;
+        for (const auto &b : bufs) total += b.second;
+        if (total == 0) return SSMD_OK;
+        SSMD_HIP(hipMalloc(reinterpret_cast<void **>(&copy), total));
+        size_t off = 0;
+        for (const auto &b : bufs) {
+            SSMD_HIP(hipMemcpyAsync(copy + off, b.first, b.second, hipMemcpyDeviceToDevice, s));
+            off += b.second;
+        }
+        return SSMD_OK;
+    }
+    int restore() {
+        if (!copy) return SSMD_OK;
+        int rc = 0;
+        size_t off = 0;
+        for (const auto &b : bufs) {
+            if (rc == 0) rc = herr(hipMemcpyAsync(b.first, copy + off, b.second, hipMemcpyDeviceToDevice, s));
+            off += b.second;
+        }
+        const int rs = herr(hipStreamSynchronize(s));
+        (void)hipFree(copy);
+        copy = nullptr;
+        return rc ? rc : rs;
+    }
+    ~GKeeper() {
+        if (copy) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(copy);
+        }
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -825,23 +872,32 @@ int ssmd_tune(const ssmd_params *p, const ssmd_fields *d, int32_t reps, void *st
     if (!fields_ready(*p, d)) return SSMD_ENOTREADY;
     if (!best_tile || !best_lz || reps < 1) return SSMD_EINVAL;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    SSMD_TRY(launch_phase(*p, *d, s));
+    GKeeper keep;
+    if (d->g_next == d->g) keep.bufs.push_back({d->g, ssmd_bytes(p, 2)});
+    SSMD_TRY(keep.save(s));
     float best = 1e30f;
     int bt = 0, bl = 0;
     const int interior = int(p->nz) - 2;
-    for (int t = 0; t < kNumTiles; ++t) {
-        const int lzs[] = {auto_lz(int(p->nx), int(p->ny), int(p->nz), t), 8, 16, 32, 64};
-        for (int lz : lzs) {
-            if (lz > interior && lz != lzs[0]) continue;
-            float ms = 0.f;
-            SSMD_TRY(time_launches(s, reps, [&]() { return launch_lattice_sel(*p, *d, 0, t, lz, s); }, &ms));
-            if (ms < best) {
-                best = ms;
-                bt = t;
-                bl = lz;
+    const int rc = [&]() -> int {
+        SSMD_TRY(launch_phase(*p, *d, s));
+        for (int t = 0; t < kNumTiles; ++t) {
+            const int lzs[] = {auto_lz(int(p->nx), int(p->ny), int(p->nz), t), 8, 16, 32, 64};
+            for (int lz : lzs) {
+                if (lz > interior && lz != lzs[0]) continue;
+                float ms = 0.f;
+                SSMD_TRY(time_launches(s, reps, [&]() { return launch_lattice_sel(*p, *d, 0, t, lz, s); }, &ms));
+                if (ms < best) {
+                    best = ms;
+                    bt = t;
+                    bl = lz;
+                }
             }
         }
-    }
+        return SSMD_OK;
+    }();
+    const int rr = keep.restore();
+    SSMD_TRY(rc);
+    SSMD_TRY(rr);
     *best_tile = bt;
     *best_lz = bl;
     return SSMD_OK;
@@ -1073,6 +1129,10 @@ int ssmd_ensemble_graph_launch(ssmd_ensemble *e, uint64_t step0, uint64_t nsteps
 int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *best_tile) {
     if (!e || !best_tile || reps < 1) return SSMD_EINVAL;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    GKeeper keep;
+    for (int j = 0; j < e->n; ++j)
+        if (e->d[j].g_next == e->d[j].g) keep.bufs.push_back({e->d[j].g, ssmd_bytes(&e->p[j], 2)});
+    SSMD_TRY(keep.save(s));
     JetSlot *tmp[2] = {nullptr, nullptr};
     int rc = 0;
     for (int par = 0; par < 2 && rc == 0; ++par)
@@ -1105,7 +1165,9 @@ int ssmd_ensemble_tune(ssmd_ensemble *e, int32_t reps, void *stream, int32_t *be
     }
     for (int par = 0; par < 2; ++par)
         if (tmp[par]) (void)hipFree(tmp[par]);
+    const int rr = keep.restore();
     if (rc != 0) return rc;
+    SSMD_TRY(rr);
     // restore the face/cell block counts of the tile in use, then switch
     uint32_t total = 0;
     SSMD_TRY(ensemble_tables(e, e->tile, e->table, &total));

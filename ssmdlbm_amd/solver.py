@@ -44,7 +44,10 @@ class Simulation:
                 continue  # never allocated, as in the reference (main.cu:55-70)
             self.t[name] = z(n)
         pdt = _POP_DTYPE[int(self.p.pop_format)]
-        self._pops = {"fa": z(q * n, dtype=pdt), "fb": z(q * n, dtype=pdt), "ga": z(4 * n), "gb": z(4 * n)}
+        # one g buffer for both g and g_next: its one reader, the phase update,
+        # runs before the lattice kernel of the same step rewrites it
+        g = z(4 * n)
+        self._pops = {"fa": z(q * n, dtype=pdt), "fb": z(q * n, dtype=pdt), "ga": g, "gb": g}
         self.t["step_counter"] = z(1, dtype=torch.int64)
         self.t["reduce"] = z(abi.REDUCE_DOUBLES, dtype=torch.float64)
         self.d = abi.Fields()
