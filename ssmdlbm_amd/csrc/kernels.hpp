@@ -46,6 +46,9 @@ struct KParams {
     // the lattice kernel stores g through the caches (k_phase reads it next)
     // when one step's g fits the 256 MB Infinity Cache, else non-temporally
     int g_cached;
+    // FP32 D3Q19 lattice kernels pull with non-temporal loads (D3Q27 always
+    // does): set where g_cached, so the f stream leaves g and phi cached
+    int nt_pull;
     int closure;                 // SSMD_CLOSURE_*
     float smag;                  // 18 Cs^2
     float strain_scale, grad_scale;
@@ -670,7 +673,7 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
     // Pull the Q populations arriving at (x, y, zz), as stored.  Each pull's
     // uniform base is computed on the scalar unit right where it is used
     // (not held, and spilled, across the loop).
-    auto pull = [&](int zz, PT (&f)[Q]) {
+    auto pull_as = [&]<bool NT>(int zz, PT (&f)[Q]) {
         const uint32_t c = cell_id(k, x, y, zz);
         long zsp = 0, zsm = 0;
         if (k.wrap_z) {
@@ -704,12 +707,30 @@ __device__ __forceinline__ void lattice_tile(const KParams &k, const typename P:
             const GChar *base = plane + shift;
             const uint32_t off = d.x > 0 ? op : (d.x < 0 ? om : o0);
             const auto *src = reinterpret_cast<const __attribute__((address_space(1))) PT *>(base + off);
-            // D3Q27 reads faster with non-temporal loads, D3Q19 slower (measured)
-            if constexpr (Q == 27)
+            if constexpr (NT)
                 f[i] = __builtin_nontemporal_load(src);
             else
                 f[i] = *src;
         });
+    };
+    // Non-temporal pulls (measured): D3Q27 reads faster with them on every
+    // box.  FP32 D3Q19 where one step's g fits the Infinity Cache
+    // (k.nt_pull): they leave g and phi there for the next phase update and
+    // lattice kernel (-7% step on 128x256x256), elsewhere they are slower
+    // (-4.7% on 256x256x1024).  16-bit D3Q19 pulls slower with them on every
+    // box, and even an untaken branch to them costs it 3%: never.
+    constexpr bool kNtAlways = Q == 27, kNtMaybe = Q != 27 && sizeof(PT) == 4;
+    auto pull = [&](int zz, PT (&f)[Q]) {
+        if constexpr (kNtAlways) {
+            pull_as.template operator()<true>(zz, f);
+        } else if constexpr (kNtMaybe) {
+            if (k.nt_pull)
+                pull_as.template operator()<true>(zz, f);
+            else
+                pull_as.template operator()<false>(zz, f);
+        } else {
+            pull_as.template operator()<false>(zz, f);
+        }
     };
     // The populations of the next D planes are in flight while plane z
     // collides: the regularized collision needs only the moments, so a

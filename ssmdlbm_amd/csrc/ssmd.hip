@@ -71,6 +71,7 @@ KParams make_kparams(const ssmd_params &p) {
     // g in moment form is 16 B per cell; cached stores measured +1.8% on an
     // 8.4 M-cell box (134 MB of g) and -1% on a 67 M-cell one (1.07 GB)
     k.g_cached = uint64_t(p.nx) * p.ny * p.nz * 16u <= (uint64_t(160) << 20) ? 1 : 0;
+    k.nt_pull = k.g_cached;
     k.closure = p.closure;
     k.smag = float(18.0 * p.cs * p.cs);
     k.strain_scale = float(p.strain_scale);
@@ -951,6 +952,7 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
         const ssmd_fields &o = e->orig[j];
         JetSlot js{};
         js.k = make_kparams(pp);
+        js.k.nt_pull = 0;  // the members' g together exceed the Infinity Cache
         js.phi = o.phi;
         js.mac = make_macro(o);
         js.narrow = mixed && narrow_member(int(pp.nx)) ? 1 : 0;
