@@ -387,7 +387,11 @@ __device__ __forceinline__ float closure_tau(const KParams &k, MP macp, const Ce
     w.y = bf16_pair(m.pzz * s, m.pxy * s);
     w.z = bf16_pair(m.pxz * s, m.pyz * s);
     w.w = bf16_pair(ph, grad * k.grad_scale);
-    if (write_inputs) *reinterpret_cast<uint4 *>(mac->feat + size_t(c) * 8) = w;
+    // non-temporal: the network pass reads these once, and cached stores of
+    // them would push g and phi out of the Infinity Cache (+0.6% on the step)
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+    if (write_inputs)
+        __builtin_nontemporal_store(U4{w.x, w.y, w.z, w.w}, reinterpret_cast<U4 *>(mac->feat + size_t(c) * 8));
     return tau;
 }
 
