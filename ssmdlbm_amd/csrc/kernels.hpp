@@ -4,7 +4,9 @@
 //   scalar fields  float[N], idx = x + nx*(y + ny*z)
 //   f, f_next      Q planes of N pop elements (f32 / f16 / bf16 deviations)
 //   g, g_next      4 planes of N floats: the D3Q7 phase populations in
-//                  moment form (g_phi, o_x, o_y, o_z), see phase_column
+//                  moment form (g_phi, o_x, o_y, o_z), see phase_column;
+//                  usually one buffer (k_phase reads g before k_lattice of
+//                  the same step writes it)
 // Populations are source-indexed ("pull"): slot [i][s] holds what leaves
 // cell s along c_i; the population arriving at cell a along c_i is slot
 // [i][a - c_i].  Every f element is read once and written once per step.
