@@ -276,6 +276,10 @@ int ssmd_quantize(int32_t pop_format, const float *in, float *out, uint64_t n, v
 int ssmd_tune(const ssmd_params *p, const ssmd_fields *d, int32_t reps, void *stream, int32_t *best_tile,
               int32_t *best_lz);
 int ssmd_tile_count(void);
+/* The automatic z-chunk (what p->lz = 0 selects) for tile shape `tile` on
+ * the box of *p: planes per lattice block, or a negative SSMD_E* code.  No
+ * reference counterpart. */
+int32_t ssmd_auto_lz(const ssmd_params *p, int32_t tile);
 int ssmd_tile_shape(int32_t tile, int32_t *tx, int32_t *ty);
 
 /* Ensembles (batching form): n independent field sets of uneven size and

@@ -133,6 +133,7 @@ SIGNATURES = {
     "ssmd_tune": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_int32, C.c_void_p, C.POINTER(C.c_int32),
                             C.POINTER(C.c_int32)]),
     "ssmd_tile_count": (C.c_int, []),
+    "ssmd_auto_lz": (C.c_int32, [C.POINTER(Params), C.c_int32]),
     "ssmd_tile_shape": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ssmd_ensemble_create": (C.c_int, [C.POINTER(Params), C.POINTER(Fields), C.c_int32, C.POINTER(C.c_void_p)]),
     "ssmd_ensemble_step": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),

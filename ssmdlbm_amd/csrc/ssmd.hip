@@ -790,6 +790,12 @@ int ssmd_quantize(int32_t fmt, const float *in, float *out, uint64_t n, void *st
 // ---------------------------------------------------------------------------
 // Autotuner
 
+int32_t ssmd_auto_lz(const ssmd_params *p, int32_t tile) {
+    SSMD_TRY(ssmd_validate(p));
+    if (tile < 0 || tile >= kNumTiles) return SSMD_EINVAL;
+    return auto_lz(int(p->nx), int(p->ny), int(p->nz), tile);
+}
+
 int ssmd_tile_count(void) { return kNumTiles; }
 
 int ssmd_tile_shape(int32_t tile, int32_t *tx, int32_t *ty) {

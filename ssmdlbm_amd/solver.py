@@ -8,6 +8,7 @@ torch stream of the device.  There is no CPU or torch compute path.
 from __future__ import annotations
 
 import ctypes as C
+import time
 from typing import Dict, Iterable, Optional
 
 import numpy as np
@@ -135,16 +136,40 @@ class Simulation:
                                                 self.stream_handle), "checkpoint_read")
         self.step_index = int(step.value)
 
-    def tune(self, reps: int = 20) -> tuple[int, int]:
-        """Autotune the lattice tile shape and z-chunk; the choice is kept in self.p."""
+    def tune(self, reps: int = 20, confirm_steps: int = 0) -> tuple[int, int]:
+        """Autotune the lattice tile shape and z-chunk; the choice is kept in self.p.
+
+        ssmd_tune times the lattice kernel alone.  With confirm_steps > 0 the
+        pick is then checked on whole steps: confirm_steps graph-replayed steps
+        at the pick and at the same tile with the automatic z-chunk, keeping
+        the faster (kernel-alone near-ties between z-chunks can go either way
+        in the step).  That advances the simulation by 2 (16 + confirm_steps)
+        steps; without it the state is left as ssmd_tune leaves it."""
         t, lz = C.c_int32(), C.c_int32()
         abi.check(self.lib.ssmd_tune(C.byref(self.p), C.byref(self.d), reps, self.stream_handle, C.byref(t),
                                      C.byref(lz)), "ssmd_tune")
-        self.p.tile, self.p.lz = t.value, lz.value
+        self._set_shape(t.value, lz.value)
+        if confirm_steps > 0:
+            auto = int(self.lib.ssmd_auto_lz(C.byref(self.p), t.value))
+            abi.check(min(auto, 0), "ssmd_auto_lz")
+            if auto != lz.value:
+                timed = {}
+                for cand in (lz.value, auto):
+                    self._set_shape(t.value, cand)
+                    self.graph_run(16)
+                    self.synchronize()
+                    t0 = time.perf_counter()
+                    self.graph_run(confirm_steps)
+                    self.synchronize()
+                    timed[cand] = time.perf_counter() - t0
+                self._set_shape(t.value, min(timed, key=timed.get))
+        return int(self.p.tile), int(self.p.lz)
+
+    def _set_shape(self, tile: int, lz: int) -> None:
+        self.p.tile, self.p.lz = tile, lz
         if self._graph is not None:  # graphs hold the old launch shape
             self.lib.ssmd_graph_destroy(self._graph)
             self._graph = None
-        return t.value, lz.value
 
     def close(self) -> None:
         if self._graph is not None:

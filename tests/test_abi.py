@@ -152,6 +152,23 @@ def test_tile_shapes():
     assert lib.ssmd_tile_shape(len(shapes), C.byref(C.c_int32()), C.byref(C.c_int32())) == -1
 
 
+def test_automatic_z_chunk():
+    """lz = 0 halves the interior (rounding up) while the launch has fewer
+    than 2048 blocks and the chunk exceeds 16 planes: 16 for the 128x2 tile
+    on 128x128x512 (64 columns x 32 chunks), 64 for 128x4 on 256x256x1024
+    (128 x 16), and 22 -> 11 planes on a box too small to reach 2048."""
+    lib = abi.load()
+    assert lib.ssmd_auto_lz(cases.JET_D3Q27.params(), 1) == 16
+    assert lib.ssmd_auto_lz(cases.JET_D3Q19_FP16.params(), 3) == 64
+    assert lib.ssmd_auto_lz(cases.JET_D3Q19_FP16.params(), 0) == 128  # 64x4: 256 columns x 8 chunks
+    small = cases.JET_D3Q19_SMALL.with_(nx=20, ny=18, nz=24).params()
+    assert lib.ssmd_auto_lz(small, 0) == 11  # 1 x 5 columns, interior 22: one halving
+    assert lib.ssmd_auto_lz(cases.JET_D3Q27.params(), lib.ssmd_tile_count()) == abi.EINVAL
+    bad = cases.JET_D3Q27.params()
+    bad.nx = 2
+    assert lib.ssmd_auto_lz(bad, 1) == abi.EINVAL
+
+
 def test_error_paths_without_device():
     lib = abi.load()
     p = cases.JET_D3Q19_SMALL.params()
