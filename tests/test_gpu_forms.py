@@ -165,6 +165,43 @@ def test_tuner_is_side_effect_free(gpu):
         np.testing.assert_array_equal(a.field(name), b.field(name))
 
 
+def test_tuner_step_check_advances_like_steps(gpu):
+    """Simulation.tune(confirm_steps) times whole graph steps at the pick and
+    at the automatic z-chunk, advancing the simulation; the state it leaves is
+    that of the same number of plain steps (launch shapes change no result)."""
+    p = cases.JET_D3Q19_SMALL.with_(nx=40, ny=32, nz=96).params()
+    lib = abi.load()
+    auto = lib.ssmd_auto_lz(p, 1)
+    assert auto != 8
+
+    class Picks8:  # the library, with ssmd_tune answering tile 1, lz 8
+        def __getattr__(self, name):
+            return getattr(lib, name)
+
+        def ssmd_tune(self, p_, d_, reps, stream, tile, lz):
+            tile._obj.value, lz._obj.value = 1, 8
+            return 0
+
+    a = Simulation(p)
+    a.init()
+    a.step(5)
+    a.lib = Picks8()
+    t, lz = a.tune(reps=1, confirm_steps=8)
+    a.lib = lib
+    assert t == 1 and lz in (8, auto)
+    assert a.step_index == 5 + 2 * (16 + 8)
+    a.step(3, macros=True)
+    b = Simulation(p)
+    b.init()
+    b.step(a.step_index - 1)
+    b.step(1, macros=True)
+    for name in MACRO:
+        np.testing.assert_array_equal(a.field(name), b.field(name), err_msg=name)
+    np.testing.assert_array_equal(a.pops_source("f"), b.pops_source("f"))
+    a.close()
+    b.close()
+
+
 def test_ensemble_tune(gpu):
     ens = Ensemble([c.params() for c in small_ensemble(n=3)])
     ens.init()
