@@ -321,6 +321,40 @@ def test_closed_box_phase_mass_conserved(gpu, q):
     assert abs(m1 / m0 - 1) <= 1e-6, (m0, m1)
 
 
+def test_closed_box_phase_mass_conserved_at_full_size(gpu):
+    """The FP32 twin of the 16-bit storage case at its own size
+    (256x256x1024 D3Q19, 67 M cells), closed and truly periodic, so the
+    large-box paths (non-temporal g stores, 4-plane phase chunks with
+    non-temporal phi stores, the tuned launch shape) keep the FP64 phase
+    mass: a droplet of radius 40 held to 1e-6 over 100 graph-replayed steps."""
+    p = cases.closed_box(256, q=19, sigma=0.02)
+    p.nz = 1024
+    sim = Simulation(p)
+    sim.init()
+    dev = sim._current("g").device
+    z = torch.arange(p.nz, device=dev, dtype=torch.float32)[:, None, None]
+    y = torch.arange(p.ny, device=dev, dtype=torch.float32)[None, :, None]
+    x = torch.arange(p.nx, device=dev, dtype=torch.float32)[None, None, :]
+    r = torch.sqrt((x - 128.3) ** 2 + (y - 127.5) ** 2 + (z - 511.2) ** 2)
+    phi = 0.5 * (1 - torch.tanh(2 * (r - 40) / 4))
+    g = sim._current("g")
+    g.zero_()
+    g[:p.cells].copy_(phi.reshape(-1))  # at rest: g_phi = phi, o = 0
+    del phi, r
+    torch.cuda.synchronize()
+    sim.step(1)
+    print("256x256x1024 closed box: tune, 100 graph steps", flush=True)
+    sim.tune(reps=2)
+    m0 = sim.phase_mass()[0]
+    sim.graph_run(100)
+    m1 = sim.phase_mass()[0]
+    assert m0 > 0.9 * 4 / 3 * np.pi * 40 ** 3
+    assert abs(m1 / m0 - 1) <= 1e-6, (m0, m1)
+    sim.close()
+    del sim, g
+    torch.cuda.empty_cache()
+
+
 # -- graphs, reductions ------------------------------------------------------------
 @pytest.mark.parametrize("q", [19, 27])
 def test_graph_replay_matches_eager_bitwise(gpu, q):
