@@ -1005,7 +1005,12 @@ __global__ void __launch_bounds__(256) k_phase_quad_batch(const JetSlot *__restr
     const uint32_t rowq = (uint32_t(k.nx) >> 2) * uint32_t(k.ny - 2), pb = (rowq + 255u) >> 8;
     const uint32_t lb = b - js.cell_block_begin, zc = lb / pb, q = (lb - zc * pb) * 256u + threadIdx.x;
     if (q >= rowq) return;
-    phase_quad_column<kEnsemblePhaseLZ, false>(k, js.g, js.phi, q, zc);
+    // phi non-temporally where the members' g together exceed the Infinity
+    // Cache (g_cached is set for the whole ensemble): +0.5% on the 32 jets
+    if (k.g_cached)
+        phase_quad_column<kEnsemblePhaseLZ, false>(k, js.g, js.phi, q, zc);
+    else
+        phase_quad_column<kEnsemblePhaseLZ, true>(k, js.g, js.phi, q, zc);
 }
 #endif  // SSMD_STEP_KERNELS
 

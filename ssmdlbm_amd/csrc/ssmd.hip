@@ -953,12 +953,18 @@ int ensemble_tables(ssmd_ensemble *e, int tile, JetSlot *dst[2], uint32_t *total
     const TileShape ts = kTiles[mixed ? 3 : tile];
     bool quad = true;
     for (int j = 0; j < e->n; ++j) quad = quad && e->p[j].nx % 4 == 0;
+    uint64_t batch_cells = 0;
+    for (int j = 0; j < e->n; ++j) batch_cells += uint64_t(e->p[j].nx) * e->p[j].ny * e->p[j].nz;
+    const bool batch_fits = batch_cells * 16u <= (uint64_t(160) << 20);  // as make_kparams' g_cached
     for (int j = 0; j < e->n; ++j) {
         const ssmd_params &pp = e->p[j];
         const ssmd_fields &o = e->orig[j];
         JetSlot js{};
         js.k = make_kparams(pp);
-        js.k.nt_pull = 0;  // the members' g together exceed the Infinity Cache
+        // the cache policies follow the whole batch: its phase update and
+        // lattice launch run over every member's g and phi
+        js.k.nt_pull = 0;
+        js.k.g_cached = batch_fits ? 1 : 0;
         js.phi = o.phi;
         js.mac = make_macro(o);
         js.narrow = mixed && narrow_member(int(pp.nx)) ? 1 : 0;
