@@ -599,21 +599,22 @@ def test_d3q27_jet_128x128x512_against_fp64(gpu):
 
 
 @pytest.mark.slow
-def test_reference_default_case_full_size_against_fp64(gpu):
-    """The reference's own case at its own size (128x256x256 D3Q19: water jet
+@pytest.mark.parametrize("q", [19, 27])
+def test_reference_default_case_full_size_against_fp64(gpu, q):
+    """The reference's own case at its own size (128x256x256, D3Q19 and D3Q27: water jet
     on y = 0, oil jet on z = 0, outflows on y = ny-1 and z = nz-1, x periodic
     as periodicX; constants.cuh:70-101), graph-replayed: every field within
     1e-5 relative L2 of the FP64 oracle after 10 steps (the force within
     1e-4, as for the small boxes)."""
-    p = cases.reference_default(19)
+    p = cases.reference_default(q)
     assert (p.nx, p.ny, p.nz) == (128, 256, 256) and p.water.enabled and p.oil.enabled
     steps = 10
-    print(f"reference default 128x256x256: {steps} GPU steps", flush=True)
+    print(f"reference default D3Q{q} 128x256x256: {steps} GPU steps", flush=True)
     sim = run_gpu(p, steps, graph=True)
     print("FP64 oracle", flush=True)
     o = run_oracle(p, steps)
     errs = {name: rel_l2(sim.field(name), o.field(name)) for name in ALL}
-    print("reference default 128x256x256, 10 steps, relative L2 vs FP64:", errs)
+    print(f"reference default D3Q{q} 128x256x256, 10 steps, relative L2 vs FP64:", errs)
     for name, e in errs.items():
         assert e < (1e-4 if name.startswith("fs") else 1e-5), (name, e)
     sim.close()
