@@ -7,6 +7,8 @@
 //     --save-every M     .bin/.vti output interval (default 1000; 0 = none, the BENCHMARK build)
 //     --pop f32|f16|bf16 population storage (default f32)
 //     --out DIR          output root (default ./bin); files go to DIR/<VS>/<ID>/
+//     --no-tune          keep the default launch shape (else ssmd_tune picks the lattice
+//                        tile and z-chunk on the t = 0 state before the graphs are made)
 //   GPU_INDEX selects the device (hostFunctions.cuh:139-172).
 //
 // Prints the metadata block, "Step N" at saves, and the final time and MLUPS
@@ -40,7 +42,7 @@ void check(int rc, const char *what) {
 void usage(const char *argv0) {
     std::cerr << "Usage: " << argv0
               << " <D3Q19|D3Q27> <ID> [--case reference|jet_small|jet_d3q27|jet_large] [--steps N]"
-                 " [--save-every M] [--pop f32|f16|bf16] [--out DIR]\n";
+                 " [--save-every M] [--pop f32|f16|bf16] [--out DIR] [--no-tune]\n";
 }
 
 // Single bottom-face jet: tau, tau_g, width, sigma, diameter, u, seed.
@@ -83,6 +85,7 @@ int main(int argc, char **argv) {
     const int q = vs == "D3Q19" ? 19 : 27;
     std::string kase = "reference", pop = "f32", out_root = "bin";
     uint64_t nsteps = 1000, save_every = 1000;
+    bool tune = true;
     for (int i = 3; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -97,6 +100,7 @@ int main(int argc, char **argv) {
         else if (a == "--save-every") save_every = std::stoull(next());
         else if (a == "--pop") pop = next();
         else if (a == "--out") out_root = next();
+        else if (a == "--no-tune") tune = false;
         else {
             usage(argv[0]);
             return 1;
@@ -152,6 +156,13 @@ int main(int argc, char **argv) {
               << "Cells:              " << uint64_t(p.nx) * p.ny * p.nz << '\n'
               << "-----------------------------------------------------------------------------\n\n";
 
+    if (tune) {  // launch shape only: the state the first step reads is unchanged
+        int32_t tile = 0, lz = 0;
+        check(ssmd_tune(&p, &d, 10, s, &tile, &lz), "tune");
+        p.tile = tile;
+        p.lz = lz;
+        std::cout << "Lattice launch shape: tile " << tile << ", z-chunk " << lz << " (ssmd_tune)\n";
+    }
     ssmd_graph *g = nullptr;
     check(ssmd_graph_create(&p, &d, s, &g), "graph");
     const char *names[] = {"phi", "rho"};  // main.cu:126-129
